@@ -1,0 +1,191 @@
+// Device functors: the load / reduce / store customisation of the schedule.
+//
+// The reference expresses these as GLSL macros (NVPRO_PYRAMID_TYPE, _LOAD,
+// _REDUCE, _REDUCE2, _REDUCE4, _LOAD_REDUCE4, _STORE, nvpro_pyramid.glsl:27-120)
+// and ships one set, sRGBA8 (srgba8_mipmap_preamble.glsl:19-128).  Here each set
+// is a struct of static device functions; the kernels are templates over it.
+//
+//   Raw            storage texel (what a level holds in memory)
+//   Smem           per-workgroup LDS tables the functor needs (may be empty)
+//   fill(s, scale) workgroup-cooperative LDS set-up; `scale` premultiplies the
+//                  decoded value (0.25 for the 2x2 box, see box() below)
+//   load<PRE>(s,t) NVPRO_PYRAMID_LOAD: Raw -> float4 linear (times 0.25 if PRE)
+//   store(s, v)    NVPRO_PYRAMID_STORE: float4 linear -> Raw
+//
+// The arithmetic on float4 (REDUCE, REDUCE2, REDUCE4) is shared: see reduce3,
+// reduce2 and box.  Every FP32 operation is written out explicitly; the library
+// is compiled with -ffp-contract=off so none is fused behind our back.
+#pragma once
+
+#include <stdint.h>
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "srgb_tables.inc"
+
+namespace nvpro_pyramid {
+
+__constant__ uint32_t gSrgbDecode[256]                        = {NVPRO_SRGB_DECODE_BITS};
+__constant__ uint32_t gSrgbEncode[2 * NVPRO_SRGB_ENC_BUCKETS] = {NVPRO_SRGB_ENCODE_BUCKETS};
+
+__device__ __forceinline__ float4 f4add(float4 a, float4 b)
+{
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+__device__ __forceinline__ float4 f4mul(float4 a, float s)
+{
+  return make_float4(a.x * s, a.y * s, a.z * s, a.w * s);
+}
+
+// NVPRO_PYRAMID_REDUCE of srgba8_mipmap_preamble.glsl:24-25, a0*v0 + a1*v1 + a2*v2,
+// with the contraction fixed as fma(a2, v2, fma(a1, v1, a0 * v0)).
+__device__ __forceinline__ float reduce3(float w0, float v0, float w1, float v1, float w2, float v2)
+{
+  return __builtin_fmaf(w2, v2, __builtin_fmaf(w1, v1, w0 * v0));
+}
+__device__ __forceinline__ float4 reduce3(float w0, float4 v0, float w1, float4 v1, float w2, float4 v2)
+{
+  return make_float4(reduce3(w0, v0.x, w1, v1.x, w2, v2.x), reduce3(w0, v0.y, w1, v1.y, w2, v2.y),
+                     reduce3(w0, v0.z, w1, v1.z, w2, v2.z), reduce3(w0, v0.w, w1, v1.w, w2, v2.w));
+}
+
+// NVPRO_PYRAMID_REDUCE2 of srgba8_mipmap_preamble.glsl:35: 0.5 * (v0 + v1).
+__device__ __forceinline__ float4 reduce2(float4 v0, float4 v1) { return f4mul(f4add(v0, v1), 0.5f); }
+
+// NVPRO_PYRAMID_REDUCE4 of srgba8_mipmap_preamble.glsl:37-38,
+// 0.25 * ((v00 + v01) + (v10 + v11)).  The reference passes the four quadrants
+// of a 2x2 square in one of two orders (nvpro_pyramid.glsl:179-188 vs
+// :364-392); `order` selects which pairs are added first:
+//   order 0: (UL + LL) + (UR + LR)      order 1: (UL + UR) + (LL + LR)
+// PRE: the inputs were already multiplied by 0.25 (exact), so the sum is the
+// result.
+template <bool PRE>
+__device__ __forceinline__ float4 box(float4 ul, float4 ur, float4 ll, float4 lr, int order)
+{
+  float4 s = order == 0 ? f4add(f4add(ul, ll), f4add(ur, lr)) : f4add(f4add(ul, ur), f4add(ll, lr));
+  return PRE ? s : f4mul(s, 0.25f);
+}
+
+// Summation order of relative level r of a fast dispatch filling `count` levels
+// (nvpro_pyramid.glsl:272-532): LOAD_REDUCE4 and the 6-level in-thread stage use
+// order 0, the subgroup-shuffle stages order 1, the shared-memory stages order 0.
+__device__ __forceinline__ int fastOrder(uint32_t count, uint32_t r)
+{
+  if(r == 1u)
+    return 0;
+  if(count <= 3u)
+    return 1;
+  if(count == 6u)
+    return (r == 3u || r == 4u) ? 1 : 0;
+  return r <= 3u ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// sRGBA8: srgba8_mipmap_preamble.glsl:19-128 with exact conversions.
+// ---------------------------------------------------------------------------
+struct Srgba8
+{
+  using Raw                       = uint32_t;
+  static constexpr uint32_t kBytes = 4;
+
+  struct Smem
+  {
+    float dec[256];                       // linear value of each code, times the fill scale
+    uint2 enc[NVPRO_SRGB_ENC_BUCKETS];    // {first float with a higher code, code at start}
+  };
+
+  __device__ static void fill(Smem& s, float scale)
+  {
+    for(uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
+      s.dec[i] = __uint_as_float(gSrgbDecode[i]) * scale;
+    for(uint32_t i = threadIdx.x; i < NVPRO_SRGB_ENC_BUCKETS; i += blockDim.x)
+      s.enc[i] = make_uint2(gSrgbEncode[2u * i], gSrgbEncode[2u * i + 1u]);
+  }
+
+  // texelFetch of the sRGB sampler (preamble:21-22): sRGB decode of RGB from the
+  // exact table, alpha = code * (1/255) (shaders/srgb.h:216).
+  template <bool PRE>
+  __device__ static float4 load(const Smem& s, uint32_t t)
+  {
+    constexpr float kAlpha = PRE ? (1.0f / 255.0f) * 0.25f : (1.0f / 255.0f);
+    return make_float4(s.dec[t & 0xffu], s.dec[(t >> 8) & 0xffu], s.dec[(t >> 16) & 0xffu],
+                       (float)(t >> 24) * kAlpha);
+  }
+
+  // srgbFromLinear (preamble:110-121), exact: the bucket of x's top bits gives the
+  // code at the bucket start and the float at which it steps up.
+  __device__ static uint32_t encodeChannel(const Smem& s, float x)
+  {
+    float xc     = fminf(fmaxf(x, 0x1p-13f), 0x1.fffffep0f);
+    uint32_t idx = (__float_as_uint(xc) >> (23u - NVPRO_SRGB_ENC_MANT_BITS))
+                   - (NVPRO_SRGB_ENC_MIN_EXP << NVPRO_SRGB_ENC_MANT_BITS);
+    uint2 e = s.enc[idx];
+    return e.y + (x >= __uint_as_float(e.x) ? 1u : 0u);
+  }
+
+  // srgbFromLinearVec (preamble:123-128); alpha = floor(fma(a, 255, 0.5)) clamped.
+  __device__ static uint32_t store(const Smem& s, float4 v)
+  {
+    uint32_t a = (uint32_t)fminf(fmaxf(__builtin_fmaf(v.w, 255.0f, 0.5f), 0.0f), 255.0f);
+    return encodeChannel(s, v.x) | encodeChannel(s, v.y) << 8 | encodeChannel(s, v.z) << 16 | a << 24;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// RGBA32F: linear float4 texels, identity load and store.
+// ---------------------------------------------------------------------------
+struct Rgba32f
+{
+  struct __attribute__((aligned(4))) Raw
+  {
+    float x, y, z, w;
+  };
+  static constexpr uint32_t kBytes = 16;
+  struct Smem
+  {
+    int unused;
+  };
+  __device__ static void fill(Smem&, float) {}
+  template <bool PRE>
+  __device__ static float4 load(const Smem&, Raw t)
+  {
+    float4 v = make_float4(t.x, t.y, t.z, t.w);
+    return PRE ? f4mul(v, 0.25f) : v;
+  }
+  __device__ static Raw store(const Smem&, float4 v) { return Raw{v.x, v.y, v.z, v.w}; }
+};
+
+// ---------------------------------------------------------------------------
+// RGBA16F: linear half4 texels; FP32 arithmetic, round-to-nearest-even on store.
+// ---------------------------------------------------------------------------
+struct Rgba16f
+{
+  struct __attribute__((aligned(4))) Raw
+  {
+    uint32_t lo, hi;  // (r | g << 16), (b | a << 16) as IEEE half bits
+  };
+  static constexpr uint32_t kBytes = 8;
+  struct Smem
+  {
+    int unused;
+  };
+  __device__ static void fill(Smem&, float) {}
+  __device__ static float h2f(uint32_t bits)
+  {
+    return __half2float(__ushort_as_half((unsigned short)bits));
+  }
+  __device__ static uint32_t f2h(float f) { return __half_as_ushort(__float2half_rn(f)); }
+  template <bool PRE>
+  __device__ static float4 load(const Smem&, Raw t)
+  {
+    float4 v = make_float4(h2f(t.lo & 0xffffu), h2f(t.lo >> 16), h2f(t.hi & 0xffffu), h2f(t.hi >> 16));
+    return PRE ? f4mul(v, 0.25f) : v;
+  }
+  __device__ static Raw store(const Smem&, float4 v)
+  {
+    return Raw{f2h(v.x) | f2h(v.y) << 16, f2h(v.z) | f2h(v.w) << 16};
+  }
+};
+
+}  // namespace nvpro_pyramid

@@ -1,0 +1,288 @@
+"""Host-side mirror of nvpro_pyramid_dispatch.hpp over the C-ABI library.
+
+Names follow the reference interface (nvpro_pyramid/nvpro_pyramid_dispatch.hpp):
+
+    NvproPyramidPipelines, NvproPyramidState       structs of :29-35 and :63-75
+    nvpro_cmd_pyramid_dispatch(...)                nvproCmdPyramidDispatch, :54-59 / :294-304
+    nvpro_cmd_pyramid_dispatch_custom(...)         the dispatcher overload, :109-188
+    DEFAULT_FAST_DISPATCHER / DEFAULT_GENERAL_DISPATCHER   :195-242 / :247-292
+
+The library is loaded from vk_compute_mipmaps_amd/lib/; importing this module
+never falls back to anything else: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libnvpro_pyramid_hip.so")
+
+INPUT_LEVEL_SHIFT = 5
+MAX_LEVELS = 16
+PIPELINE_GENERAL = 0
+PIPELINE_FAST = 1
+FORMAT_SRGBA8 = 0
+FORMAT_RGBA16F = 1
+FORMAT_RGBA32F = 2
+ARITH_FP32 = 0
+ARITH_PACKED_FP16 = 1
+BYTES_PER_TEXEL = {FORMAT_SRGBA8: 4, FORMAT_RGBA16F: 8, FORMAT_RGBA32F: 16}
+
+# Every symbol include/nvpro_pyramid_hip.h declares.
+EXPORTED_SYMBOLS = (
+    "nvproPyramidCmdDispatch",
+    "nvproPyramidDefaultFastDispatcher",
+    "nvproPyramidFastDispatcherEx",
+    "nvproPyramidDefaultGeneralDispatcher",
+    "nvproPyramidDispatch",
+    "nvproPyramidDispatchCustom",
+    "nvproPyramidPlan",
+    "nvproPyramidFullLevelCount",
+    "nvproPyramidPackedLayout",
+    "nvproPyramidBytesPerTexel",
+    "nvproPyramidGetLastError",
+)
+
+
+class NvproPyramidError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"nvpro_pyramid error {code}: {message}")
+        self.code = code
+
+
+class NvproPyramidPipelines(ctypes.Structure):
+    _fields_ = [("format", ctypes.c_uint32), ("fastPipeline", ctypes.c_uint32),
+                ("arithmetic", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class NvproPyramidImage(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("levelCount", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("levelOffsets", ctypes.POINTER(ctypes.c_uint64)),
+                ("rowPitches", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class NvproPyramidState(ctypes.Structure):
+    _fields_ = [("currentLevel", ctypes.c_uint32), ("remainingLevels", ctypes.c_uint32),
+                ("currentX", ctypes.c_uint32), ("currentY", ctypes.c_uint32)]
+
+
+class NvproPyramidStep(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in (
+        "pipeline", "inputLevel", "levelCount", "pushConstant", "groupCountX",
+        "inputWidth", "inputHeight", "reserved")]
+
+    def as_tuple(self):
+        return (self.pipeline, self.inputLevel, self.levelCount, self.groupCountX)
+
+    def __repr__(self):
+        kind = "F" if self.pipeline == PIPELINE_FAST else "G"
+        return f"{kind}({self.inputLevel},{self.levelCount},{self.groupCountX})"
+
+
+DISPATCHER = ctypes.CFUNCTYPE(ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(NvproPyramidState))
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"HIP library {LIB_PATH} is not built; run `python -m vk_compute_mipmaps_amd.build`")
+        L = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+        u32, vp = ctypes.c_uint32, ctypes.c_void_p
+        L.nvproPyramidFullLevelCount.restype = u32
+        L.nvproPyramidFullLevelCount.argtypes = [u32, u32]
+        L.nvproPyramidBytesPerTexel.restype = u32
+        L.nvproPyramidBytesPerTexel.argtypes = [u32]
+        L.nvproPyramidGetLastError.restype = ctypes.c_char_p
+        L.nvproPyramidGetLastError.argtypes = []
+        L.nvproPyramidPackedLayout.restype = ctypes.c_int
+        L.nvproPyramidPackedLayout.argtypes = [u32, u32, u32, u32, ctypes.POINTER(ctypes.c_uint64),
+                                               ctypes.POINTER(ctypes.c_uint64)]
+        L.nvproPyramidPlan.restype = ctypes.c_int
+        L.nvproPyramidPlan.argtypes = [ctypes.POINTER(NvproPyramidPipelines), u32, u32, u32, vp, vp,
+                                       ctypes.POINTER(NvproPyramidStep), u32, ctypes.POINTER(u32)]
+        L.nvproPyramidDispatch.restype = ctypes.c_int
+        L.nvproPyramidDispatch.argtypes = [vp, ctypes.POINTER(NvproPyramidPipelines),
+                                           ctypes.POINTER(NvproPyramidImage), u32]
+        L.nvproPyramidDispatchCustom.restype = ctypes.c_int
+        L.nvproPyramidDispatchCustom.argtypes = [vp, ctypes.POINTER(NvproPyramidPipelines),
+                                                 ctypes.POINTER(NvproPyramidImage), u32, vp, vp]
+        L.nvproPyramidCmdDispatch.restype = ctypes.c_int
+        L.nvproPyramidCmdDispatch.argtypes = [vp, u32, u32, u32]
+        for name in ("nvproPyramidDefaultFastDispatcher", "nvproPyramidDefaultGeneralDispatcher"):
+            getattr(L, name).restype = u32
+            getattr(L, name).argtypes = [vp, ctypes.POINTER(NvproPyramidState)]
+        L.nvproPyramidFastDispatcherEx.restype = u32
+        L.nvproPyramidFastDispatcherEx.argtypes = [vp, ctypes.POINTER(NvproPyramidState), u32, u32]
+        _lib = L
+    return _lib
+
+
+def last_error() -> str:
+    return lib().nvproPyramidGetLastError().decode()
+
+
+def _check(code: int):
+    if code != 0:
+        raise NvproPyramidError(code, last_error())
+
+
+def dispatcher_address(fn) -> int:
+    """Address of a library dispatcher or of a DISPATCHER-wrapped Python callable."""
+    return ctypes.cast(fn, ctypes.c_void_p).value
+
+
+def default_fast_dispatcher() -> int:
+    return dispatcher_address(lib().nvproPyramidDefaultFastDispatcher)
+
+
+def default_general_dispatcher() -> int:
+    return dispatcher_address(lib().nvproPyramidDefaultGeneralDispatcher)
+
+
+def full_level_count(width: int, height: int) -> int:
+    return lib().nvproPyramidFullLevelCount(width, height)
+
+
+def level_dims(width: int, height: int, level: int):
+    for _ in range(level):
+        width = (width >> 1) | (width == 1)
+        height = (height >> 1) | (height == 1)
+    return width, height
+
+
+def packed_layout(width: int, height: int, levels: int = 0, bytes_per_texel: int = 4):
+    """MipmapStorage layout: (list of per-level byte offsets, total bytes)."""
+    levels = levels or full_level_count(width, height)
+    offs = (ctypes.c_uint64 * levels)()
+    total = ctypes.c_uint64()
+    _check(lib().nvproPyramidPackedLayout(width, height, levels, bytes_per_texel, offs, ctypes.byref(total)))
+    return [int(o) for o in offs], int(total.value)
+
+
+def chain_bytes(width: int, height: int, levels: int = 0, bytes_per_texel: int = 4) -> int:
+    """Algorithmic bytes of a chain: base read once plus every output level written once."""
+    return packed_layout(width, height, levels, bytes_per_texel)[1]
+
+
+def pipelines(fmt: int = FORMAT_SRGBA8, fast: bool = True, arithmetic: int = ARITH_FP32):
+    return NvproPyramidPipelines(fmt, 1 if fast else 0, arithmetic, 0)
+
+
+def plan(width: int, height: int, mip_levels: int = 0, fast: bool = True, fmt: int = FORMAT_SRGBA8,
+         general_dispatcher=None, fast_dispatcher=None):
+    pipes = pipelines(fmt, fast)
+    steps = (NvproPyramidStep * 64)()
+    n = ctypes.c_uint32()
+    _check(lib().nvproPyramidPlan(ctypes.byref(pipes), width, height, mip_levels,
+                                  _fn_ptr(general_dispatcher), _fn_ptr(fast_dispatcher), steps, 64,
+                                  ctypes.byref(n)))
+    return [steps[i] for i in range(min(n.value, 64))]
+
+
+def _fn_ptr(fn):
+    if fn is None:
+        return None
+    if isinstance(fn, int):
+        return ctypes.c_void_p(fn)
+    return ctypes.cast(fn, ctypes.c_void_p)
+
+
+def make_image(data_ptr: int, width: int, height: int, level_count: int = 0, level_offsets=None,
+               row_pitches=None):
+    img = NvproPyramidImage()
+    img.data = data_ptr
+    img.width, img.height = width, height
+    img.levelCount = level_count
+    img.reserved = 0
+    keep = []
+    if level_offsets is not None:
+        arr = (ctypes.c_uint64 * len(level_offsets))(*level_offsets)
+        img.levelOffsets = arr
+        keep.append(arr)
+    if row_pitches is not None:
+        arr = (ctypes.c_uint32 * len(row_pitches))(*row_pitches)
+        img.rowPitches = arr
+        keep.append(arr)
+    img._keep = keep
+    return img
+
+
+def nvpro_cmd_pyramid_dispatch(stream: int, pipes: NvproPyramidPipelines, image: NvproPyramidImage,
+                               mip_levels: int = 0):
+    """nvproCmdPyramidDispatch over a HIP stream; raises NvproPyramidError."""
+    _check(lib().nvproPyramidDispatch(ctypes.c_void_p(stream), ctypes.byref(pipes), ctypes.byref(image),
+                                      mip_levels))
+
+
+def nvpro_cmd_pyramid_dispatch_custom(stream: int, pipes: NvproPyramidPipelines, image: NvproPyramidImage,
+                                      mip_levels: int, general_dispatcher, fast_dispatcher):
+    _check(lib().nvproPyramidDispatchCustom(ctypes.c_void_p(stream), ctypes.byref(pipes), ctypes.byref(image),
+                                            mip_levels, _fn_ptr(general_dispatcher), _fn_ptr(fast_dispatcher)))
+
+
+POISON_SRGBA8 = 0xFFFF00FF  # magenta, opaque (bytes FF 00 FF FF), as the reference's test clears
+
+
+class DeviceMipChain:
+    """A packed MipmapStorage chain in device memory (one hipMalloc)."""
+
+    def __init__(self, width: int, height: int, fmt: int = FORMAT_SRGBA8, levels: int = 0):
+        from . import hip_rt
+
+        self.rt = hip_rt
+        self.width, self.height, self.fmt = width, height, fmt
+        self.levels = levels or full_level_count(width, height)
+        self.bpt = BYTES_PER_TEXEL[fmt]
+        self.offsets, self.nbytes = packed_layout(width, height, self.levels, self.bpt)
+        self.ptr = hip_rt.malloc(self.nbytes)
+        self.image = make_image(self.ptr, width, height, self.levels)
+
+    def upload_level0(self, level0: np.ndarray):
+        a = np.ascontiguousarray(level0)
+        assert a.nbytes == self.width * self.height * self.bpt
+        self.rt.memcpy_htod(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+
+    def upload(self, chain: np.ndarray):
+        a = np.ascontiguousarray(chain)
+        assert a.nbytes == self.nbytes
+        self.rt.memcpy_htod(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+
+    def poison(self, word: int = POISON_SRGBA8, stream: int = 0):
+        """Fill levels >= 1 with a poison pattern (unwritten texels stay visible)."""
+        start = self.offsets[1] if self.levels > 1 else self.nbytes
+        if self.nbytes > start:
+            self.rt.memset_d32_async(self.ptr + start, word, (self.nbytes - start) // 4, stream)
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.nbytes, dtype=np.uint8)
+        self.rt.memcpy_dtoh(out.ctypes.data_as(ctypes.c_void_p), self.ptr, self.nbytes)
+        return out
+
+    def dispatch(self, stream: int = 0, fast: bool = True, mip_levels: int = 0, arithmetic: int = ARITH_FP32,
+                 general_dispatcher=None, fast_dispatcher=None):
+        pipes = pipelines(self.fmt, fast, arithmetic)
+        if general_dispatcher is None and fast_dispatcher is None:
+            nvpro_cmd_pyramid_dispatch(stream, pipes, self.image, mip_levels)
+        else:
+            nvpro_cmd_pyramid_dispatch_custom(stream, pipes, self.image, mip_levels,
+                                              general_dispatcher or default_general_dispatcher(),
+                                              fast_dispatcher or default_fast_dispatcher())
+
+    def free(self):
+        if self.ptr:
+            self.rt.free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
