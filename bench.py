@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark: µs per 4096x4096 sRGBA8 full mip chain on one MI355X.
+
+A step is one nvproPyramidDispatch of the full 13-level chain of a 4096x4096
+sRGBA8 image (plan F(0,6) F(6,6)).  Chains rotate over >= 1 GiB of distinct
+device buffers so the 256 MB Infinity Cache cannot hold the working set.
+Timing uses HIP events on the dispatch stream around exactly --steps
+dispatches, after --warmup untimed ones.
+
+Printed (one JSON line): the metric, per-chain time, chains/s, the roofline of
+the dominant kernel (the 6-level fast-pipeline launch over level 0), and the
+reference's CPU generator (mipmap_storage.hpp:246-414, restated in C) timed on
+the host as a baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "µs per 4096² sRGBA8 chain + %HBM roofline; batched mip-chains/s at p99 latency"
+HBM_PEAK_GBS = 8000.0
+W = H = 4096
+SEED = 2
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--rotate-bytes", type=float, default=1.1e9)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--only-first-step", action="store_true",
+                    help="dispatch only the 6-level fast step over level 0 (for profiling)")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path has no multi-GPU leg",
+              file=sys.stderr)
+        sys.exit(2)
+    import ctypes
+
+    import numpy as np
+
+    from vk_compute_mipmaps_amd import hip_rt, pyramid as P, synth
+
+    hip_rt.check(hip_rt.rt().hipSetDevice(0), "hipSetDevice")
+    stream = hip_rt.stream_create()
+    level0 = synth.srgba8(W, H, SEED)
+    chain_bytes = P.chain_bytes(W, H)
+    n_buf = max(2, int(np.ceil(args.rotate_bytes / chain_bytes)))
+    chains = [P.DeviceMipChain(W, H) for _ in range(n_buf)]
+    chains[0].upload_level0(level0)
+    for c in chains[1:]:
+        hip_rt.memcpy_dtod_async(c.ptr, chains[0].ptr, W * H * 4, stream)
+    for c in chains:
+        c.poison(stream=stream)
+    hip_rt.stream_sync(stream)
+
+    pipes = P.pipelines()
+    levels = P.full_level_count(W, H)
+    first = P.plan(W, H)[0]
+    step_levels = first.inputLevel + first.levelCount + 1  # mipLevels covering the first step only
+    mip = step_levels if args.only_first_step else 0
+
+    def dispatch(i, mip_levels=mip):
+        P.nvpro_cmd_pyramid_dispatch(stream, pipes, chains[i % n_buf].image, mip_levels)
+
+    for i in range(args.warmup):
+        dispatch(i)
+    hip_rt.stream_sync(stream)
+
+    # Timed region: exactly --steps dispatches.
+    e0, e1 = hip_rt.event_create(), hip_rt.event_create()
+    hip_rt.device_sync()
+    hip_rt.event_record(e0, stream)
+    for i in range(args.steps):
+        dispatch(i)
+    hip_rt.event_record(e1, stream)
+    hip_rt.event_sync(e1)
+    hip_rt.device_sync()
+    total_ms = hip_rt.event_elapsed_ms(e0, e1)
+    us_per_chain = total_ms * 1000.0 / args.steps
+
+    # Per-chain distribution (separate pass, an event pair per chain).
+    reps = max(50, min(args.steps, 400))
+    evs = [hip_rt.event_create() for _ in range(reps + 1)]
+    hip_rt.event_record(evs[0], stream)
+    for i in range(reps):
+        dispatch(i)
+        hip_rt.event_record(evs[i + 1], stream)
+    hip_rt.event_sync(evs[-1])
+    per = sorted(hip_rt.event_elapsed_ms(evs[i], evs[i + 1]) * 1000.0 for i in range(reps))
+    p99 = per[min(len(per) - 1, int(round(0.99 * (len(per) - 1))))]
+
+    # Dominant kernel: the first step alone (6-level fast pipeline over level 0).
+    k_reps = max(50, min(args.steps, 400))
+    hip_rt.event_record(evs[0], stream)
+    for i in range(k_reps):
+        dispatch(i, step_levels)
+    hip_rt.event_record(evs[1], stream)
+    hip_rt.event_sync(evs[1])
+    k_us = hip_rt.event_elapsed_ms(evs[0], evs[1]) * 1000.0 / k_reps
+    k_bytes = P.packed_layout(W, H, step_levels)[1]
+    k_gbs = k_bytes / (k_us * 1e-6) / 1e9
+
+    # Achievable copy bandwidth for reference (device-to-device, 1 GiB).
+    cp_bytes = 1 << 30
+    src, dst = hip_rt.malloc(cp_bytes), hip_rt.malloc(cp_bytes)
+    hip_rt.memcpy_dtod_async(dst, src, cp_bytes, stream)
+    hip_rt.stream_sync(stream)
+    hip_rt.event_record(evs[0], stream)
+    for _ in range(5):
+        hip_rt.memcpy_dtod_async(dst, src, cp_bytes, stream)
+    hip_rt.event_record(evs[1], stream)
+    hip_rt.event_sync(evs[1])
+    copy_gbs = 2 * cp_bytes * 5 / (hip_rt.event_elapsed_ms(evs[0], evs[1]) * 1e-3) / 1e9
+    hip_rt.free(src)
+    hip_rt.free(dst)
+
+    for e in evs + [e0, e1]:
+        hip_rt.event_destroy(e)
+    for c in chains:
+        c.free()
+    hip_rt.stream_destroy(stream)
+
+    traffic = None
+    tpath = os.path.join(REPO, "profiles", "fast_pipeline_4096_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(level0, args.cpu_seconds)
+
+    chain_total_bytes = chain_bytes
+    out = {
+        "metric": METRIC,
+        "value": round(us_per_chain, 3),
+        "unit": "us",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms / args.steps, 6),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {
+            "workload": "4096x4096 sRGBA8 full chain, 13 levels, plan F(0,6) F(6,6)"
+            if not args.only_first_step else "4096x4096 sRGBA8, first fast step only F(0,6)",
+            "input": f"seeded value noise + uniform noise, seed {SEED}",
+            "rotation_buffers": n_buf,
+            "rotation_bytes": n_buf * chain_bytes,
+        },
+        "chains_per_s": round(1e6 / us_per_chain, 1),
+        "latency_us": {"min": round(per[0], 3), "median": round(statistics.median(per), 3),
+                       "p99": round(p99, 3)},
+        "chain_algorithmic_bytes": chain_total_bytes,
+        "chain_GBps": round(chain_total_bytes / (us_per_chain * 1e-6) / 1e9, 1),
+        "chain_roofline_frac": round(chain_total_bytes / (us_per_chain * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "fastPipelineKernel<Srgba8,3> (F(0,6) over level 0)",
+            "achieved": round(k_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(k_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel_us": round(k_us, 3),
+            "algorithmic_bytes_per_launch": k_bytes,
+        },
+        "copy_GBps": round(copy_gbs, 1),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(level0, seconds):
+    """The reference CPU generator (restated in oracle/orc.c) on the same 4096^2
+    input, one thread, repeated for about `seconds` of CPU work."""
+    import numpy as np
+
+    from oracle import orc
+    from vk_compute_mipmaps_amd import pyramid as P
+
+    offs, total = P.packed_layout(W, H)
+    chain = np.zeros(total, np.uint8)
+    chain[: level0.nbytes] = level0.reshape(-1)
+    times = []
+    t_end = time.perf_counter() + seconds
+    while True:
+        t0 = time.perf_counter()
+        orc.reference_cpu(chain, W, H, threads=1)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() > t_end or len(times) >= 20:
+            break
+    us = statistics.median(times) * 1e6
+    return {"value": round(us, 1), "unit": "us per 4096x4096 sRGBA8 chain", "cores": 1, "kind": "port",
+            "sample": f"{len(times)} full 4096x4096 chains (mipmap_storage.hpp generateMipmaps restated in C, "
+                      f"1 thread); median"}
+
+
+if __name__ == "__main__":
+    main()

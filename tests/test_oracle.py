@@ -1,0 +1,453 @@
+"""Pins the CPU oracle (oracle/orc.c) before it is trusted as the GPU checker.
+
+The reference holds no golden images, so each oracle function is checked
+against something independent of its own formula:
+  * sRGB decode / encode: 60-digit mpmath evaluations of shaders/srgb.h:18-28 and
+    srgba8_mipmap_preamble.glsl:110-121 (exhaustive over the 256 codes and the
+    255 code thresholds);
+  * the planner: the dispatch plans the reference's docs state
+    (docs/test_images.txt, docs/strategy.md.html) and hand-derived plans;
+  * the 2x2 box: numpy's mean over 2x2 blocks in float64;
+  * the NP2 3x3 filter: an exact-rational (fractions) brute force of the
+    weights (n - y, n, y + 1) / (2n + 1) from the NP2 paper the reference cites
+    (mipmap_storage.hpp:337-338), and energy conservation (each source texel's
+    total weight is dst/src, so a level's mean equals its source's mean);
+  * invariants: constant images stay constant through every plan;
+  * envelopes: the reference CPU generator's published worst delta (2 LSB
+    opaque, 4-5 LSB alpha: demo_app/rtx3090.json);
+  * sensitivity: planted summation-order and weight mistakes change results.
+"""
+import struct
+from fractions import Fraction
+
+import mpmath
+import numpy as np
+import pytest
+
+from tests import helpers as H
+from vk_compute_mipmaps_amd import pyramid as P
+from vk_compute_mipmaps_amd import synth
+
+mpmath.mp.dps = 60
+
+
+def f32(x):
+    return struct.unpack("<f", struct.pack("<f", x))[0]
+
+
+def f32_bits(x):
+    return struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def bits_f32(b):
+    return struct.unpack("<f", struct.pack("<I", b))[0]
+
+
+def mp_decode(c):
+    u = mpmath.mpf(c) / 255
+    if u <= mpmath.mpf("0.04045"):
+        return u * 25 / 323
+    return ((200 * u + 11) / 211) ** mpmath.mpf("2.4")
+
+
+def mp_encode(x):
+    xm = mpmath.mpf(x)
+    if not xm > 0:
+        return 0
+    if xm <= mpmath.mpf("0.0031308"):
+        s = mpmath.mpf("12.92") * xm
+    else:
+        s = mpmath.mpf("1.055") * xm ** (mpmath.mpf(5) / 12) - mpmath.mpf("0.055")
+    u = s * 255 + mpmath.mpf("0.5")
+    return int(min(255, max(0, mpmath.floor(u))))
+
+
+# ---------------------------------------------------------------------------
+# sRGB conversions
+# ---------------------------------------------------------------------------
+
+def test_decode_is_correctly_rounded(orc):
+    L = orc.lib()
+    prev = -1.0
+    for c in range(256):
+        got = L.orc_srgb_decode_f32(c)
+        exact = mp_decode(c)
+        # The nearest float: |got - exact| <= |neighbour - exact|.
+        b = f32_bits(got)
+        for nb in (b - 1, b + 1):
+            if nb >= 0:
+                assert abs(mpmath.mpf(got) - exact) <= abs(mpmath.mpf(bits_f32(nb)) - exact), c
+        assert got > prev
+        prev = got
+    assert L.orc_srgb_decode_f32(0) == 0.0 and L.orc_srgb_decode_f32(255) == 1.0
+
+
+def test_encode_thresholds_are_exact(orc):
+    """Each threshold T[k] is the first float whose exact encode reaches k."""
+    L = orc.lib()
+    for k in range(1, 256):
+        t = L.orc_srgb_threshold(k)
+        assert mp_encode(t) >= k, k
+        assert mp_encode(bits_f32(f32_bits(t) - 1)) < k, k
+        assert L.orc_srgb_encode_exact(t) == mp_encode(t)
+        assert L.orc_srgb_encode_exact(bits_f32(f32_bits(t) - 1)) == mp_encode(bits_f32(f32_bits(t) - 1))
+
+
+def test_encode_known_values(orc):
+    L = orc.lib()
+    assert L.orc_srgb_encode_exact(0.0) == 0
+    assert L.orc_srgb_encode_exact(-1.0) == 0
+    assert L.orc_srgb_encode_exact(1.0) == 255
+    assert L.orc_srgb_encode_exact(2.0) == 255
+    # 0.5 linear is 0.735357 sRGB: 0.735357 * 255 + 0.5 = 188.02
+    assert L.orc_srgb_encode_exact(0.5) == 188
+    # linear branch: 0.001 * 12.92 * 255 + 0.5 = 3.79
+    assert L.orc_srgb_encode_exact(0.001) == 3
+
+
+def test_roundtrip_every_code(orc):
+    L = orc.lib()
+    for c in range(256):
+        assert L.orc_srgb_encode_exact(L.orc_srgb_decode_f32(c)) == c
+        assert L.orc_alpha_encode_f32(L.orc_alpha_decode_f32(c)) == c
+        assert L.orc_alpha_encode_f64(c / 255.0) == c
+
+
+def test_alpha_encode_rounds_half_up(orc):
+    L = orc.lib()
+    assert L.orc_alpha_encode_f64((10 + 0.5) / 255.0) == 11
+    assert L.orc_alpha_encode_f64((10 + 0.49) / 255.0) == 10
+    assert L.orc_alpha_encode_f32(0.0) == 0 and L.orc_alpha_encode_f32(1.0) == 255
+
+
+def test_product_tables_match_oracle(orc):
+    """The generated tables the kernels use equal the oracle's independent ones."""
+    import re
+
+    path = P.PKG_DIR + "/csrc/srgb_tables.inc"
+    text = open(path).read()
+
+    def block(name):
+        body = text.split(f"#define {name} \\")[1].split("\n\n")[0]
+        return [int(t.rstrip("u"), 0) for t in re.findall(r"0x[0-9a-f]+u|\b\d+u", body)]
+
+    L = orc.lib()
+    dec = block("NVPRO_SRGB_DECODE_BITS")
+    assert dec == [f32_bits(L.orc_srgb_decode_f32(c)) for c in range(256)]
+    thr = block("NVPRO_SRGB_THRESHOLD_BITS")
+    assert thr[1:] == [f32_bits(L.orc_srgb_threshold(k)) for k in range(1, 256)]
+    buckets = block("NVPRO_SRGB_ENCODE_BUCKETS")
+    min_exp = int(re.search(r"NVPRO_SRGB_ENC_MIN_EXP (\d+)u", text).group(1))
+    mant = int(re.search(r"NVPRO_SRGB_ENC_MANT_BITS (\d+)u", text).group(1))
+    n = len(buckets) // 2
+    for i in range(n):
+        t, lo = buckets[2 * i], buckets[2 * i + 1]
+        start = (min_exp << 23) + (i << (23 - mant))
+        end = start + (1 << (23 - mant))
+        assert lo == L.orc_srgb_encode_exact(bits_f32(start))
+        steps = [k for k in range(1, 256) if start <= thr[k] < end]
+        assert len(steps) <= 1
+        assert t == (thr[steps[0]] if steps else 0x7FC00000)
+        # Every float of the bucket encodes to lo or lo + 1 by the stored threshold.
+        for b in (start, end - 1, (start + end) // 2):
+            x = bits_f32(b)
+            want = L.orc_srgb_encode_exact(x)
+            got = lo + (1 if (t != 0x7FC00000 and b >= t) else 0)
+            assert got == want
+
+
+# ---------------------------------------------------------------------------
+# Planner
+# ---------------------------------------------------------------------------
+
+F, G = 1, 0
+APPENDIX = {
+    (1024, 1024): [(F, 0, 6, 256), (F, 6, 4, 1)],
+    (4096, 4096): [(F, 0, 6, 4096), (F, 6, 6, 1)],
+    (4095, 2047): [(G, 0, 2, 8192), (G, 2, 2, 512), (G, 4, 2, 32), (G, 6, 2, 2), (G, 8, 2, 1), (G, 10, 1, 1)],
+    (1920, 1080): [(F, 0, 3, 2025), (G, 3, 2, 40), (G, 5, 2, 2), (G, 7, 2, 1), (G, 9, 1, 1)],
+    (2047, 4097): [(G, 0, 2, 8192), (G, 2, 2, 512), (G, 4, 2, 32), (G, 6, 2, 2), (G, 8, 2, 1), (G, 10, 2, 1)],
+    (16384, 16384): [(F, 0, 6, 65536), (F, 6, 6, 16), (F, 12, 2, 1)],
+    (8192, 8192): [(F, 0, 6, 16384), (F, 6, 6, 4), (G, 12, 1, 1)],
+    (2048, 2048): [(F, 0, 6, 1024), (F, 6, 5, 1)],
+    (2560, 1440): [(F, 0, 5, 3600), (G, 5, 2, 6), (G, 7, 2, 1), (G, 9, 2, 1)],
+    (3840, 2160): [(F, 0, 4, 8100), (G, 4, 2, 40), (G, 6, 2, 2), (G, 8, 2, 1), (G, 10, 1, 1)],
+    (4094, 4094): [(G, 0, 2, 16384), (G, 2, 2, 1024), (G, 4, 2, 64), (G, 6, 2, 4), (G, 8, 2, 1), (G, 10, 1, 1)],
+    (2052, 2052): [(F, 0, 2, 4113), (G, 2, 2, 256), (F, 4, 6, 4), (G, 10, 1, 1)],
+    (1080, 4096): [(F, 0, 3, 4320), (G, 3, 2, 80), (G, 5, 2, 4), (F, 7, 3, 1), (G, 10, 2, 1)],
+}
+
+
+@pytest.mark.parametrize("size", sorted(APPENDIX))
+def test_plan_known_answers(size, orc):
+    w, h = size
+    want = APPENDIX[size]
+    assert [s.as_tuple() for s in orc.plan(w, h)] == want
+    assert [s.as_tuple() for s in P.plan(w, h)] == want
+
+
+def test_plan_facts_stated_by_reference_docs(orc):
+    """docs/test_images.txt: 1080p -> 3-level fast case (:10-11), 1440p -> 5 (:16-17),
+    4k -> 4 (:38-39), 2052 -> 2 (:58-59), 4094 -> the default fast dispatcher declines
+    (divisible by 2, not 4; :22-23, strategy.md.html:473-479); strategy.md.html:332-335:
+    2048^2 and 4096^2 in two dispatches."""
+    first = lambda w, h: orc.plan(w, h)[0].as_tuple()[:3]
+    assert first(1920, 1080) == (F, 0, 3)
+    assert first(2560, 1440) == (F, 0, 5)
+    assert first(3840, 2160) == (F, 0, 4)
+    assert first(2052, 2052) == (F, 0, 2)
+    assert first(4094, 4094)[0] == G
+    assert len(orc.plan(2048, 2048)) == 2 and len(orc.plan(4096, 4096)) == 2
+    # levels_1_6 (divisibility 2) takes a 1-level fast step on 4094.
+    assert orc.plan(4094, 4094, 0, True, 2, 6)[0].as_tuple()[:3] == (F, 0, 1)
+
+
+def test_plan_matches_product_planner_sweep(orc):
+    rng = np.random.default_rng(1)
+    sizes = [(int(a), int(b)) for a, b in rng.integers(1, 5000, (300, 2))]
+    sizes += [(1, 1), (1, 2), (2, 1), (1, 65535), (65535, 1), (3, 5), (64, 64), (65535, 65535)]
+    for w, h in sizes:
+        full = P.full_level_count(w, h)
+        assert full == orc.full_levels(w, h)
+        for fast in (True, False):
+            for mip in {0, full, max(1, full // 2), min(full, 2)}:
+                want = [s.as_tuple() for s in orc.plan(w, h, mip, fast)]
+                got = [s.as_tuple() for s in P.plan(w, h, mip, fast)] if mip != 1 else []
+                assert got == want, (w, h, fast, mip)
+
+
+def test_packed_layout_matches_mipmap_storage(orc):
+    for w, h in [(1, 1), (7, 3), (4096, 4096), (4095, 2047), (1, 300)]:
+        offs, total = orc.packed_offsets(w, h)
+        poffs, ptotal = P.packed_layout(w, h, 0, 4)
+        assert [int(o) * 4 for o in offs] == poffs and total * 4 == ptotal
+        # brute force: sum of level areas
+        levels, acc = [], 0
+        lw, lh = w, h
+        while True:
+            levels.append(acc)
+            acc += lw * lh
+            if lw == 1 and lh == 1:
+                break
+            lw, lh = max(1, lw >> 1), max(1, lh >> 1)
+        assert [int(o) for o in offs] == levels and total == acc
+
+
+# ---------------------------------------------------------------------------
+# Filters
+# ---------------------------------------------------------------------------
+
+def linear_chain(w, h, level0):
+    offs, total = P.packed_layout(w, h, 0, 1)
+    chain = np.zeros((total, 4), np.float64)
+    chain[: w * h] = level0.reshape(-1, 4)
+    return chain
+
+
+def test_box_filter_matches_numpy_mean(orc):
+    w, h = 64, 32
+    rng = np.random.default_rng(2)
+    l0 = rng.random((h, w, 4))
+    out = orc.pyramid_linear_f64(linear_chain(w, h, l0), w, h)
+    offs, _ = P.packed_layout(w, h, 0, 1)
+    prev = l0
+    for l in range(1, P.full_level_count(w, h)):
+        lw, lh = P.level_dims(w, h, l)
+        if prev.shape[0] % 2 == 0 and prev.shape[1] % 2 == 0:
+            want = prev.reshape(lh, 2, lw, 2, 4).mean(axis=(1, 3))
+        else:  # one edge reached 1: pairs along the other
+            want = prev.reshape(lh, -1, lw, prev.shape[1] // lw if lw else 1, 4).mean(axis=(1, 3))
+        got = out[offs[l]: offs[l] + lw * lh].reshape(lh, lw, 4)
+        np.testing.assert_allclose(got, want, rtol=0, atol=1e-15)
+        prev = want
+
+
+def np2_bruteforce(src, dw, dh):
+    """Exact-rational NP2 filter: weights (n - y, n, y + 1) / (2n + 1) per odd axis,
+    1/2, 1/2 per even axis, copy for an axis of length 1."""
+    sh, sw = len(src), len(src[0])
+
+    def taps(s, n, d):
+        if s == 1:
+            return [(0, Fraction(1))]
+        if s % 2 == 0:
+            return [(2 * d, Fraction(1, 2)), (2 * d + 1, Fraction(1, 2))]
+        den = 2 * n + 1
+        return [(2 * d, Fraction(n - d, den)), (2 * d + 1, Fraction(n, den)), (2 * d + 2, Fraction(d + 1, den))]
+
+    out = []
+    for y in range(dh):
+        row = []
+        for x in range(dw):
+            acc = Fraction(0)
+            for yy, wy in taps(sh, dh, y):
+                for xx, wx in taps(sw, dw, x):
+                    acc += wy * wx * src[yy][xx]
+            row.append(acc)
+        out.append(row)
+    return out
+
+
+@pytest.mark.parametrize("w,h", [(5, 3), (7, 7), (1, 5), (9, 1), (6, 5), (11, 4), (3, 3)])
+def test_np2_filter_matches_exact_rational_weights(w, h, orc):
+    rng = np.random.default_rng(w * 31 + h)
+    vals = rng.integers(0, 1000, (h, w))
+    l0 = np.repeat(vals[..., None].astype(np.float64), 4, axis=2)
+    out = orc.pyramid_linear_f64(linear_chain(w, h, l0), w, h)
+    offs, _ = P.packed_layout(w, h, 0, 1)
+    src = [[Fraction(int(v)) for v in row] for row in vals]
+    for l in range(1, P.full_level_count(w, h)):
+        lw, lh = P.level_dims(w, h, l)
+        exact = np2_bruteforce(src, lw, lh)
+        got = out[offs[l]: offs[l] + lw * lh, 0].reshape(lh, lw)
+        for y in range(lh):
+            for x in range(lw):
+                assert abs(got[y, x] - float(exact[y][x])) <= 1e-9 * max(1.0, abs(float(exact[y][x])))
+        src = exact
+
+
+@pytest.mark.parametrize("w,h", [(37, 23), (4095, 7), (1, 77), (129, 128), (255, 255)])
+def test_np2_energy_conservation(w, h, orc):
+    rng = np.random.default_rng(w + h)
+    l0 = rng.random((h, w, 4))
+    out = orc.pyramid_linear_f64(linear_chain(w, h, l0), w, h)
+    offs, _ = P.packed_layout(w, h, 0, 1)
+    means = []
+    for l in range(P.full_level_count(w, h)):
+        lw, lh = P.level_dims(w, h, l)
+        means.append(out[offs[l]: offs[l] + lw * lh].mean(axis=0))
+    for m in means[1:]:
+        np.testing.assert_allclose(m, means[0], rtol=1e-12)
+
+
+@pytest.mark.parametrize("w,h", [(64, 64), (33, 17), (4, 12), (1, 9), (255, 3), (96, 200)])
+def test_constant_images_stay_constant(w, h, orc):
+    for code, alpha in [(0, 255), (1, 7), (128, 128), (200, 0), (255, 255), (11, 254)]:
+        l0 = np.zeros((h, w, 4), np.uint8)
+        l0[..., :3] = code
+        l0[..., 3] = alpha
+        chain = H.host_chain(w, h, l0)
+        for fast in (True, False):
+            steps = orc.plan(w, h, 0, fast)
+            for precision in ("f32", "f64"):
+                out = orc.pyramid_srgba8(chain, w, h, steps, 0, precision).reshape(-1, 4)
+                assert (out[:, :3] == code).all() and (out[:, 3] == alpha).all(), (code, alpha, fast, precision)
+        ref = orc.reference_cpu(chain, w, h).reshape(-1, 4)
+        assert (ref[:, :3] == code).all()
+
+
+# ---------------------------------------------------------------------------
+# Emulators against each other and against the reference CPU generator
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("w,h,seed", [(257, 129, 9), (1024, 1024, 1), (1920, 1080, 4), (4095, 2047, 3)])
+def test_fp32_emulator_within_one_lsb_of_fp64(w, h, seed, orc):
+    l0 = H.level0_srgba8(w, h, seed)
+    chain = H.host_chain(w, h, l0)
+    steps = orc.plan(w, h)
+    a = orc.pyramid_srgba8(chain, w, h, steps, 0, "f32")
+    b = orc.pyramid_srgba8(chain, w, h, steps, 0, "f64")
+    worst, rgb_exact, _ = H.rgb_alpha_stats(a, b, w, h)
+    assert worst <= 1
+    assert rgb_exact >= 0.999
+
+
+@pytest.mark.parametrize("w,h,seed", [(1024, 1024, 1), (1920, 1080, 4), (4095, 2047, 3), (2047, 2047, 13)])
+def test_reference_cpu_envelope(w, h, seed, orc):
+    """The reference CPU generator re-quantises every level, truncates alpha and
+    uses powf; its published worst delta against the GPU is 2 (opaque) and 4-5
+    (alpha) (demo_app/rtx3090.json)."""
+    for opaque, bound in ((True, 2), (False, 5)):
+        l0 = H.level0_srgba8(w, h, seed, opaque)
+        chain = H.host_chain(w, h, l0)
+        a = orc.pyramid_srgba8(chain, w, h, orc.plan(w, h))
+        ref = orc.reference_cpu(chain, w, h)
+        worst = orc.compare(a, ref, w, h)[0]
+        assert worst <= bound, (opaque, worst)
+        assert orc.compare(ref, orc.reference_cpu(chain, w, h, threads=4), w, h)[0] == 0
+
+
+@pytest.mark.parametrize("w,h", [(1024, 1024), (1920, 1080), (512, 768)])
+def test_fast_and_general_only_plans_agree(w, h, orc):
+    chain = H.host_chain(w, h, H.level0_srgba8(w, h, 5))
+    a = orc.pyramid_srgba8(chain, w, h, orc.plan(w, h, 0, True))
+    b = orc.pyramid_srgba8(chain, w, h, orc.plan(w, h, 0, False))
+    assert orc.compare(a, b, w, h)[0] <= 1
+
+
+def test_planted_order_mistake_changes_results(orc):
+    """A flipped 2x2 summation order changes stored bytes: an exact comparison
+    against the oracle detects such a mistake in a kernel."""
+    w = h = 1024
+    chain = H.host_chain(w, h, H.level0_srgba8(w, h, 1))
+    steps = orc.plan(w, h)
+    good = orc.pyramid_srgba8(chain, w, h, steps)
+    L = orc.lib()
+    flag = __import__("ctypes").c_int.in_dll(L, "orc_debug_flags")
+    try:
+        flag.value = 1
+        bad = orc.pyramid_srgba8(chain, w, h, steps)
+    finally:
+        flag.value = 0
+    worst, _, ndiff, _ = orc.compare(good, bad, w, h)
+    assert worst >= 1 and ndiff > 0
+
+
+def test_planted_weight_mistake_changes_results(orc):
+    w, h = 4095, 2047
+    chain = H.host_chain(w, h, H.level0_srgba8(w, h, 3))
+    steps = orc.plan(w, h)
+    good = orc.pyramid_srgba8(chain, w, h, steps)
+    L = orc.lib()
+    flag = __import__("ctypes").c_int.in_dll(L, "orc_debug_flags")
+    try:
+        flag.value = 2
+        bad = orc.pyramid_srgba8(chain, w, h, steps)
+    finally:
+        flag.value = 0
+    assert orc.compare(good, bad, w, h)[2] > 0
+
+
+# ---------------------------------------------------------------------------
+# Synthetic inputs
+# ---------------------------------------------------------------------------
+
+def _mix(z):
+    z = (z + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return z ^ (z >> 31)
+
+
+def _hash(seed, s, i, j):
+    return _mix(_mix(_mix(seed ^ (s << 56)) ^ i) ^ j)
+
+
+def test_synthetic_srgba8_matches_its_spec():
+    """Pure-Python restatement of the generator spec for a few texels."""
+    seed, w, h = 7, 200, 130
+    img = synth.srgba8(w, h, seed)
+    D = 1 << 18
+    for (x, y) in [(0, 0), (63, 0), (64, 1), (199, 129), (100, 77), (5, 128)]:
+        i, j, mx, my = x >> 6, y >> 6, x & 63, y & 63
+        sx, sy = mx * mx * (192 - 2 * mx), my * my * (192 - 2 * my)
+        for c in range(3):
+            L = lambda a, b: _hash(seed, c, a, b) >> 56
+            top = L(i, j) * (D - sx) + L(i + 1, j) * sx
+            bot = L(i, j + 1) * (D - sx) + L(i + 1, j + 1) * sx
+            v = (top * (D - sy) + bot * sy + (1 << 35)) >> 36
+            noise = ((_hash(seed, 4 + c, x, y) >> 32) % 33) - 16
+            assert img[y, x, c] == min(255, max(0, v + noise))
+        assert img[y, x, 3] == _hash(seed, 7, x, y) >> 56
+    band = synth.srgba8(w, h, seed, rows=(40, 90))
+    np.testing.assert_array_equal(band, img[40:90])
+
+
+def test_mixed_sizes_distribution():
+    sizes = synth.mixed_sizes(512, 8)
+    assert len(sizes) == 512
+    odd = sum(1 for a, b in sizes if a % 2 and b % 2)
+    even = sum(1 for a, b in sizes if a % 2 == 0 and b % 2 == 0)
+    assert odd + even == 512
+    assert 0.3 < odd / 512 < 0.5
+    assert all(64 <= a <= 4096 and 64 <= b <= 4096 for a, b in sizes)

@@ -21,7 +21,9 @@ LIB_PATH = os.path.join(LIB_DIR, "libnvpro_pyramid_hip.so")
 INCLUDE = os.path.join(REPO_DIR, "include")
 
 SOURCES = [
-    "pyramid_kernels.hip",
+    "fast_pipeline.hip",
+    "general_pipeline.hip",
+    "tail_pipeline.hip",
     "nvpro_pyramid_dispatch.cpp",
 ]
 

@@ -5,16 +5,17 @@
 // and ships one set, sRGBA8 (srgba8_mipmap_preamble.glsl:19-128).  Here each set
 // is a struct of static device functions; the kernels are templates over it.
 //
-//   Raw            storage texel (what a level holds in memory)
-//   Smem           per-workgroup LDS tables the functor needs (may be empty)
-//   fill(s, scale) workgroup-cooperative LDS set-up; `scale` premultiplies the
-//                  decoded value (0.25 for the 2x2 box, see box() below)
-//   load<PRE>(s,t) NVPRO_PYRAMID_LOAD: Raw -> float4 linear (times 0.25 if PRE)
-//   store(s, v)    NVPRO_PYRAMID_STORE: float4 linear -> Raw
+//   Raw                  storage texel (what a level holds in memory)
+//   Tables               per-workgroup LDS tables (may be empty)
+//   fill<BS>(t, scale)   workgroup-cooperative set-up by BS threads; `scale`
+//                        premultiplies decoded values (0.25 for the 2x2 box)
+//   load<_, PRE>(t, raw) NVPRO_PYRAMID_LOAD: Raw -> float4 linear; table-driven
+//                        formats fold PRE into the fill scale, others multiply
+//   store(t, v)          NVPRO_PYRAMID_STORE: float4 linear -> Raw
 //
-// The arithmetic on float4 (REDUCE, REDUCE2, REDUCE4) is shared: see reduce3,
-// reduce2 and box.  Every FP32 operation is written out explicitly; the library
-// is compiled with -ffp-contract=off so none is fused behind our back.
+// The arithmetic on float4 (REDUCE, REDUCE2, REDUCE4) is shared:
+// reduce3, reduce2 and box.  Every FP32 operation is written out; the library is
+// compiled with -ffp-contract=off so none is fused behind our back.
 #pragma once
 
 #include <stdint.h>
@@ -26,8 +27,8 @@
 
 namespace nvpro_pyramid {
 
-__constant__ uint32_t gSrgbDecode[256]                        = {NVPRO_SRGB_DECODE_BITS};
-__constant__ uint32_t gSrgbEncode[2 * NVPRO_SRGB_ENC_BUCKETS] = {NVPRO_SRGB_ENCODE_BUCKETS};
+static __constant__ uint32_t gSrgbDecode[256]                        = {NVPRO_SRGB_DECODE_BITS};
+static __constant__ uint32_t gSrgbEncode[2 * NVPRO_SRGB_ENC_BUCKETS] = {NVPRO_SRGB_ENCODE_BUCKETS};
 
 __device__ __forceinline__ float4 f4add(float4 a, float4 b)
 {
@@ -70,7 +71,7 @@ __device__ __forceinline__ float4 box(float4 ul, float4 ur, float4 ll, float4 lr
 // Summation order of relative level r of a fast dispatch filling `count` levels
 // (nvpro_pyramid.glsl:272-532): LOAD_REDUCE4 and the 6-level in-thread stage use
 // order 0, the subgroup-shuffle stages order 1, the shared-memory stages order 0.
-__device__ __forceinline__ int fastOrder(uint32_t count, uint32_t r)
+__host__ __device__ __forceinline__ int fastOrder(uint32_t count, uint32_t r)
 {
   if(r == 1u)
     return 0;
@@ -86,49 +87,82 @@ __device__ __forceinline__ int fastOrder(uint32_t count, uint32_t r)
 // ---------------------------------------------------------------------------
 struct Srgba8
 {
-  using Raw                       = uint32_t;
+  using Raw                        = uint32_t;
   static constexpr uint32_t kBytes = 4;
 
-  struct Smem
+  // LDS tables: the decoded value of every code for RGB and for alpha (both
+  // times the fill scale), and the encode buckets.
+  struct Tables
   {
-    float dec[256];                       // linear value of each code, times the fill scale
-    uint2 enc[NVPRO_SRGB_ENC_BUCKETS];    // {first float with a higher code, code at start}
+    float dec[256];
+    float adec[256];
+    uint2 enc[NVPRO_SRGB_ENC_BUCKETS];  // {first float with a higher code, code at start}
   };
 
-  __device__ static void fill(Smem& s, float scale)
+  template <int BS>
+  __device__ static void fill(Tables& t, float scale)
   {
-    for(uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
-      s.dec[i] = __uint_as_float(gSrgbDecode[i]) * scale;
-    for(uint32_t i = threadIdx.x; i < NVPRO_SRGB_ENC_BUCKETS; i += blockDim.x)
-      s.enc[i] = make_uint2(gSrgbEncode[2u * i], gSrgbEncode[2u * i + 1u]);
+    // Issue every global load before any LDS store.
+    constexpr int kEnc = (NVPRO_SRGB_ENC_BUCKETS + BS - 1) / BS;
+    constexpr int kDec = (256 + BS - 1) / BS;
+    uint2 e[kEnc];
+    float d[kDec];
+#pragma unroll
+    for(int k = 0; k < kEnc; ++k)
+    {
+      const uint32_t i = threadIdx.x + k * BS;
+      e[k] = i < NVPRO_SRGB_ENC_BUCKETS ? make_uint2(gSrgbEncode[2u * i], gSrgbEncode[2u * i + 1u]) : make_uint2(0, 0);
+    }
+#pragma unroll
+    for(int k = 0; k < kDec; ++k)
+    {
+      const uint32_t i = threadIdx.x + k * BS;
+      d[k] = i < 256u ? __uint_as_float(gSrgbDecode[i]) : 0.0f;
+    }
+#pragma unroll
+    for(int k = 0; k < kDec; ++k)
+    {
+      const uint32_t i = threadIdx.x + k * BS;
+      if(i < 256u)
+      {
+        t.dec[i]  = d[k] * scale;
+        t.adec[i] = (float)i * (1.0f / 255.0f) * scale;  // shaders/srgb.h:216, then the exact scale
+      }
+    }
+#pragma unroll
+    for(int k = 0; k < kEnc; ++k)
+    {
+      const uint32_t i = threadIdx.x + k * BS;
+      if(i < NVPRO_SRGB_ENC_BUCKETS)
+        t.enc[i] = e[k];
+    }
   }
 
-  // texelFetch of the sRGB sampler (preamble:21-22): sRGB decode of RGB from the
-  // exact table, alpha = code * (1/255) (shaders/srgb.h:216).
+  // texelFetch of the sRGB sampler (preamble:21-22): sRGB decode of RGB and the
+  // alpha code * (1/255) (shaders/srgb.h:216), each a table lookup.
   template <bool PRE>
-  __device__ static float4 load(const Smem& s, uint32_t t)
+  __device__ static float4 load(const Tables& t, uint32_t texel)
   {
-    constexpr float kAlpha = PRE ? (1.0f / 255.0f) * 0.25f : (1.0f / 255.0f);
-    return make_float4(s.dec[t & 0xffu], s.dec[(t >> 8) & 0xffu], s.dec[(t >> 16) & 0xffu],
-                       (float)(t >> 24) * kAlpha);
+    return make_float4(t.dec[texel & 0xffu], t.dec[(texel >> 8) & 0xffu], t.dec[(texel >> 16) & 0xffu],
+                       t.adec[texel >> 24]);
   }
 
   // srgbFromLinear (preamble:110-121), exact: the bucket of x's top bits gives the
   // code at the bucket start and the float at which it steps up.
-  __device__ static uint32_t encodeChannel(const Smem& s, float x)
+  __device__ static uint32_t encodeChannel(const Tables& t, float x)
   {
     float xc     = fminf(fmaxf(x, 0x1p-13f), 0x1.fffffep0f);
     uint32_t idx = (__float_as_uint(xc) >> (23u - NVPRO_SRGB_ENC_MANT_BITS))
                    - (NVPRO_SRGB_ENC_MIN_EXP << NVPRO_SRGB_ENC_MANT_BITS);
-    uint2 e = s.enc[idx];
+    uint2 e = t.enc[idx];
     return e.y + (x >= __uint_as_float(e.x) ? 1u : 0u);
   }
 
   // srgbFromLinearVec (preamble:123-128); alpha = floor(fma(a, 255, 0.5)) clamped.
-  __device__ static uint32_t store(const Smem& s, float4 v)
+  __device__ static uint32_t store(const Tables& t, float4 v)
   {
     uint32_t a = (uint32_t)fminf(fmaxf(__builtin_fmaf(v.w, 255.0f, 0.5f), 0.0f), 255.0f);
-    return encodeChannel(s, v.x) | encodeChannel(s, v.y) << 8 | encodeChannel(s, v.z) << 16 | a << 24;
+    return encodeChannel(t, v.x) | encodeChannel(t, v.y) << 8 | encodeChannel(t, v.z) << 16 | a << 24;
   }
 };
 
@@ -142,18 +176,24 @@ struct Rgba32f
     float x, y, z, w;
   };
   static constexpr uint32_t kBytes = 16;
-  struct Smem
+  struct Tables
   {
     int unused;
   };
-  __device__ static void fill(Smem&, float) {}
+  template <int BS>
+  __device__ static void fill(Tables&, float)
+  {
+  }
   template <bool PRE>
-  __device__ static float4 load(const Smem&, Raw t)
+  __device__ static float4 load(const Tables&, Raw t)
   {
     float4 v = make_float4(t.x, t.y, t.z, t.w);
     return PRE ? f4mul(v, 0.25f) : v;
   }
-  __device__ static Raw store(const Smem&, float4 v) { return Raw{v.x, v.y, v.z, v.w}; }
+  __device__ static Raw store(const Tables&, float4 v)
+  {
+    return Raw{v.x, v.y, v.z, v.w};
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -166,23 +206,23 @@ struct Rgba16f
     uint32_t lo, hi;  // (r | g << 16), (b | a << 16) as IEEE half bits
   };
   static constexpr uint32_t kBytes = 8;
-  struct Smem
+  struct Tables
   {
     int unused;
   };
-  __device__ static void fill(Smem&, float) {}
-  __device__ static float h2f(uint32_t bits)
+  template <int BS>
+  __device__ static void fill(Tables&, float)
   {
-    return __half2float(__ushort_as_half((unsigned short)bits));
   }
+  __device__ static float h2f(uint32_t bits) { return __half2float(__ushort_as_half((unsigned short)bits)); }
   __device__ static uint32_t f2h(float f) { return __half_as_ushort(__float2half_rn(f)); }
   template <bool PRE>
-  __device__ static float4 load(const Smem&, Raw t)
+  __device__ static float4 load(const Tables&, Raw t)
   {
     float4 v = make_float4(h2f(t.lo & 0xffffu), h2f(t.lo >> 16), h2f(t.hi & 0xffffu), h2f(t.hi >> 16));
     return PRE ? f4mul(v, 0.25f) : v;
   }
-  __device__ static Raw store(const Smem&, float4 v)
+  __device__ static Raw store(const Tables&, float4 v)
   {
     return Raw{f2h(v.x) | f2h(v.y) << 16, f2h(v.z) | f2h(v.w) << 16};
   }

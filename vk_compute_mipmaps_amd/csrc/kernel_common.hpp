@@ -1,0 +1,212 @@
+// Helpers shared by the HIP kernels: vector row access, level addressing and
+// the general pipeline's sample filter.
+#pragma once
+
+#include "functors.hpp"
+#include "pyramid_internal.hpp"
+
+namespace nvpro_pyramid {
+
+template <int NW>
+struct __attribute__((aligned(4))) Words
+{
+  uint32_t w[NW];
+};
+
+// Load / store NBYTES (a multiple of 4) at a 4-byte aligned address with the
+// widest dword-aligned vector accesses (global_load_dwordx4 for 16 bytes).
+template <int NBYTES>
+__device__ __forceinline__ void loadWords(const uint8_t* p, uint32_t* out)
+{
+  constexpr int NW = NBYTES / 4;
+  if constexpr(NW >= 4)
+  {
+#pragma unroll
+    for(int i = 0; i < NW / 4; ++i)
+    {
+      Words<4> v = reinterpret_cast<const Words<4>*>(p)[i];
+#pragma unroll
+      for(int k = 0; k < 4; ++k)
+        out[4 * i + k] = v.w[k];
+    }
+  }
+  else
+  {
+    Words<NW> v = *reinterpret_cast<const Words<NW>*>(p);
+#pragma unroll
+    for(int k = 0; k < NW; ++k)
+      out[k] = v.w[k];
+  }
+}
+
+template <int NBYTES>
+__device__ __forceinline__ void storeWords(uint8_t* p, const uint32_t* in)
+{
+  constexpr int NW = NBYTES / 4;
+  if constexpr(NW >= 4)
+  {
+#pragma unroll
+    for(int i = 0; i < NW / 4; ++i)
+    {
+      Words<4> v;
+#pragma unroll
+      for(int k = 0; k < 4; ++k)
+        v.w[k] = in[4 * i + k];
+      reinterpret_cast<Words<4>*>(p)[i] = v;
+    }
+  }
+  else
+  {
+    Words<NW> v;
+#pragma unroll
+    for(int k = 0; k < NW; ++k)
+      v.w[k] = in[k];
+    *reinterpret_cast<Words<NW>*>(p) = v;
+  }
+}
+
+template <class F, int N>
+__device__ __forceinline__ void loadRow(const uint8_t* p, typename F::Raw (&out)[N])
+{
+  static_assert(sizeof(typename F::Raw) == F::kBytes, "raw texel size");
+  uint32_t w[N * F::kBytes / 4];
+  loadWords<N * F::kBytes>(p, w);
+  __builtin_memcpy(out, w, sizeof w);
+}
+
+template <class F, int N>
+__device__ __forceinline__ void storeRow(uint8_t* p, const typename F::Raw (&in)[N])
+{
+  uint32_t w[N * F::kBytes / 4];
+  __builtin_memcpy(w, in, sizeof w);
+  storeWords<N * F::kBytes>(p, w);
+}
+
+template <class F>
+__device__ __forceinline__ typename F::Raw loadTexel(const uint8_t* p)
+{
+  typename F::Raw t[1];
+  loadRow<F, 1>(p, t);
+  return t[0];
+}
+
+template <class F>
+__device__ __forceinline__ void storeTexel(uint8_t* p, typename F::Raw v)
+{
+  typename F::Raw t[1] = {v};
+  storeRow<F, 1>(p, t);
+}
+
+__device__ __forceinline__ uint8_t* texelPtr(const ImageDesc& img, uint32_t level, uint32_t x, uint32_t y,
+                                             uint32_t bytes)
+{
+  const LevelDesc& d = img.lv[level];
+  return img.base + d.offset + (uint64_t)y * d.pitch + (uint64_t)x * bytes;
+}
+
+// Even bits of v packed together (Morton decode of one coordinate).
+__device__ __forceinline__ uint32_t compactEven(uint32_t v)
+{
+  v &= 0x55555555u;
+  v = (v | (v >> 1)) & 0x33333333u;
+  v = (v | (v >> 2)) & 0x0f0f0f0fu;
+  v = (v | (v >> 4)) & 0x00ff00ffu;
+  v = (v | (v >> 8)) & 0x0000ffffu;
+  return v;
+}
+
+// kernelSizeFromInputSize_ (nvpro_pyramid.glsl:557-561).
+__host__ __device__ __forceinline__ int kernelSize(uint32_t dim) { return dim == 1u ? 1 : (int)(2u | (dim & 1u)); }
+
+// Vertical weights of reduceStoreSample_ (nvpro_pyramid.glsl:581-586) for
+// destination coordinate d of a level n texels long.
+struct Weights3
+{
+  float w0, w1, w2;
+};
+__device__ __forceinline__ Weights3 weights3(uint32_t n, uint32_t d)
+{
+  const float nf  = (float)n;
+  const float rcp = 1.0f / (2.0f * nf + 1.0f);
+  const float w0  = rcp * (nf - (float)d);
+  const float w1  = rcp * nf;
+  return Weights3{w0, w1, (1.0f - w0) - w1};
+}
+
+// reduceStoreSample_ (nvpro_pyramid.glsl:575-656) without the store: the sample
+// of destination texel (dx, dy) of a (dw x dh) level from a kx x ky window whose
+// texel at (column j, row i) `at(j, i)` returns.  Vertical first, then horizontal.
+template <class At>
+__device__ __forceinline__ float4 generalSample(At at, int kx, int ky, uint32_t dw, uint32_t dh, uint32_t dx,
+                                                uint32_t dy)
+{
+  const Weights3 wy = weights3(dh, dy);
+  float4 h[3];
+#pragma unroll
+  for(int j = 0; j < 3; ++j)
+  {
+    if(j < kx)
+    {
+      if(ky == 3)
+        h[j] = reduce3(wy.w0, at(j, 0), wy.w1, at(j, 1), wy.w2, at(j, 2));
+      else if(ky == 2)
+        h[j] = reduce2(at(j, 0), at(j, 1));
+      else
+        h[j] = at(j, 0);
+    }
+  }
+  if(kx == 3)
+  {
+    const Weights3 wx = weights3(dw, dx);
+    return reduce3(wx.w0, h[0], wx.w1, h[1], wx.w2, h[2]);
+  }
+  if(kx == 2)
+    return reduce2(h[0], h[1]);
+  return h[0];
+}
+
+// DPP row shift-left: lane i reads lane i + n of its 16-lane row (bound_ctrl: 0
+// beyond the row).  Only the lowest lane of each reduced group is consumed.
+template <int N>
+__device__ __forceinline__ float dppShl(float v)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x100 + N, 0xf, 0xf,
+                                                                true));
+}
+template <int N>
+__device__ __forceinline__ float4 dppShl(float4 v)
+{
+  return make_float4(dppShl<N>(v.x), dppShl<N>(v.y), dppShl<N>(v.z), dppShl<N>(v.w));
+}
+
+__device__ __forceinline__ float4 shflXor(float4 v, int mask)
+{
+  return make_float4(__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64), __shfl_xor(v.z, mask, 64),
+                     __shfl_xor(v.w, mask, 64));
+}
+
+// One cross-lane 2x2 reduction level: the group leader (lowest lane) combines its
+// value with lanes +mx (x neighbour), +my (y neighbour) and +mx+my.
+template <int MX>
+__device__ __forceinline__ float4 crossLaneBox(float4 v, int order)
+{
+  constexpr int MY = 2 * MX;
+  if constexpr(MY < 16)
+  {
+    if(order == 0)
+    {
+      float4 s = f4add(v, dppShl<MY>(v));
+      return f4mul(f4add(s, dppShl<MX>(s)), 0.25f);
+    }
+    float4 s = f4add(v, dppShl<MX>(v));
+    return f4mul(f4add(s, dppShl<MY>(s)), 0.25f);
+  }
+  else
+  {
+    const int first = order == 0 ? MY : MX, second = order == 0 ? MX : MY;
+    float4 s        = f4add(v, shflXor(v, first));
+    return f4mul(f4add(s, shflXor(s, second)), 0.25f);
+  }
+}
+
+}  // namespace nvpro_pyramid

@@ -373,12 +373,11 @@ extern "C" int nvproPyramidDispatchCustom(hipStream_t stream, const NvproPyramid
     return 0;
   if(int e = runDispatchLoop(&cmd, generalDispatcher, fastDispatcher))
     return e;
-  for(uint32_t i = 0; i < cmd.stepCount; ++i)
+  if(cmd.stepCount != 0)
   {
-    const NvproPyramidStep& s = cmd.steps[i];
-    hipError_t e              = launchStep(stream, desc, s.pipeline, s.inputLevel, s.levelCount);
+    hipError_t e = executePlan(stream, desc, cmd.steps, cmd.stepCount);
     if(e != hipSuccess)
-      return fail(e, "%s: launch of step %u failed: %s", who, i, hipGetErrorString(e));
+      return fail(e, "%s: kernel launch failed: %s", who, hipGetErrorString(e));
   }
   gLastError[0] = 0;
   return 0;

@@ -29,10 +29,31 @@ struct ImageDesc
   LevelDesc lv[NVPRO_PYRAMID_MAX_LEVELS];
 };
 
-// Launch one dispatch of the fast or general pipeline over levels
-// inputLevel+1 .. inputLevel+levelCount.  The caller has validated the step.
-hipError_t launchStep(hipStream_t stream, const ImageDesc& img, uint32_t pipeline,
-                      uint32_t inputLevel, uint32_t levelCount);
+// The steps a single tail launch runs (see tail_pipeline.hip).
+struct TailStep
+{
+  uint32_t pipeline, inputLevel, levelCount, pad;
+};
+struct TailPlan
+{
+  uint32_t count, pad[3];
+  TailStep steps[NVPRO_PYRAMID_MAX_LEVELS];
+};
+
+// A step whose first output level has at most kTailMaxTexels texels, and every
+// step after it, runs in the one-workgroup tail kernel of kTailBS threads.
+constexpr uint32_t kTailMaxTexels = 1024;
+constexpr int kTailBS             = 256;
+
+// Per-pipeline launchers.  The caller has validated every step.
+hipError_t launchFast(hipStream_t stream, const ImageDesc& img, uint32_t inputLevel, uint32_t levelCount,
+                      uint32_t wavesPerCU);
+hipError_t launchGeneral(hipStream_t stream, const ImageDesc& img, uint32_t inputLevel, uint32_t levelCount);
+hipError_t launchTail(hipStream_t stream, const ImageDesc& img, const TailPlan& plan);
+
+// Launch a validated plan: per-step kernels for the large steps, then one tail
+// launch for the small remainder.
+hipError_t executePlan(hipStream_t stream, const ImageDesc& img, const NvproPyramidStep* steps, uint32_t count);
 
 // Number of compute units of the current device (cached per device).
 uint32_t deviceComputeUnits();
