@@ -825,3 +825,202 @@ int orc_pyramid_rgba32f_f32(float* chain, uint32_t w, uint32_t h, uint32_t mipLe
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* RGBA16F: IEEE binary16 texels                                              */
+/* ------------------------------------------------------------------------ */
+
+/* binary16 -> double, exact. */
+double orc_half_to_double(uint16_t h)
+{
+  const int sign = h >> 15, exp = (h >> 10) & 31, man = h & 1023;
+  double v;
+  if(exp == 0)
+    v = ldexp((double)man, -24);
+  else if(exp == 31)
+    v = man ? NAN : INFINITY;
+  else
+    v = ldexp((double)(man | 1024), exp - 25);
+  return sign ? -v : v;
+}
+
+/* double -> binary16, round to nearest even, with subnormals and overflow to inf. */
+uint16_t orc_half_from_double(double x)
+{
+  if(isnan(x))
+    return 0x7e00;
+  uint16_t sign = signbit(x) ? 0x8000 : 0;
+  double a = fabs(x);
+  if(a >= 65520.0) /* rounds to inf */
+    return sign | 0x7c00;
+  if(a < ldexp(1.0, -25)) /* below half the smallest subnormal (ties to even: 0) */
+    return sign;
+  int e;
+  frexp(a, &e); /* a = m * 2^e, m in [0.5, 1) */
+  int q = e - 11; /* quantum exponent for an 11-bit significand */
+  if(q < -24)
+    q = -24; /* subnormal quantum */
+  double scaled = ldexp(a, -q); /* exact: power-of-two scaling */
+  double r      = nearbyint(scaled); /* default rounding mode: to nearest even */
+  uint32_t m    = (uint32_t)r;
+  if(q == -24 && m < 1024)
+    return sign | (uint16_t)m; /* subnormal (or rounds up to the smallest normal below) */
+  if(m == 2048)
+  {
+    m = 1024;
+    q += 1;
+  }
+  int biased = q + 25; /* m in [1024, 2047], value = m * 2^q */
+  if(biased >= 31)
+    return sign | 0x7c00;
+  return sign | (uint16_t)(biased << 10) | (uint16_t)(m & 1023);
+}
+
+float orc_half_to_float(uint16_t h) { return (float)orc_half_to_double(h); }
+
+/* A binary16 operation: exact in double (two halves' sum, product with 0.25 or
+ * 0.5, are exact in double), then one rounding to half. */
+static double h16(double x) { return orc_half_to_double(orc_half_from_double(x)); }
+
+typedef struct { double c[4]; } h4; /* values that are exactly representable halves */
+
+static h4 box_h16(h4 ul, h4 ur, h4 ll, h4 lr, int order)
+{
+  h4 o;
+  for(int c = 0; c < 4; ++c)
+  {
+    double s = order == 0 ? h16(h16(ul.c[c] + ll.c[c]) + h16(ur.c[c] + lr.c[c]))
+                          : h16(h16(ul.c[c] + ur.c[c]) + h16(ll.c[c] + lr.c[c]));
+    o.c[c] = h16(s * 0.25);
+  }
+  return o;
+}
+
+/* reduceStoreSample_ in PACKED_FP16 arithmetic: 2-tap stages in half precision,
+ * 3-tap stages widened to FP32 (weights as in general_f32), rounded to half. */
+static h4 general_h16(const h4* src, uint32_t sw, uint32_t dw, uint32_t dh, uint32_t x, uint32_t y, int kx, int ky)
+{
+  float n   = (float)dh;
+  float rcp = 1.0f / (2.0f * n + 1.0f);
+  float w0  = rcp * (n - (float)y);
+  float w1  = rcp * n;
+  float w2  = (1.0f - w0) - w1;
+  h4 hcol[3];
+  for(int j = 0; j < kx; ++j)
+  {
+    const h4* col = src + (uint64_t)(2u * y) * sw + 2u * x + (uint32_t)j;
+    for(int c = 0; c < 4; ++c)
+    {
+      if(ky == 3)
+        hcol[j].c[c] = h16((double)reduce3_f32(w0, (float)col[0].c[c], w1, (float)col[sw].c[c], w2,
+                                               (float)col[2u * sw].c[c]));
+      else if(ky == 2)
+        hcol[j].c[c] = h16(h16(col[0].c[c] + col[sw].c[c]) * 0.5);
+      else
+        hcol[j].c[c] = col[0].c[c];
+    }
+  }
+  h4 o;
+  if(kx == 3)
+  {
+    n   = (float)dw;
+    rcp = 1.0f / (2.0f * n + 1.0f);
+    w0  = rcp * (n - (float)x);
+    w1  = rcp * n;
+    w2  = (1.0f - w0) - w1;
+    for(int c = 0; c < 4; ++c)
+      o.c[c] = h16((double)reduce3_f32(w0, (float)hcol[0].c[c], w1, (float)hcol[1].c[c], w2, (float)hcol[2].c[c]));
+  }
+  else if(kx == 2)
+  {
+    for(int c = 0; c < 4; ++c)
+      o.c[c] = h16(h16(hcol[0].c[c] + hcol[1].c[c]) * 0.5);
+  }
+  else
+    o = hcol[0];
+  return o;
+}
+
+int orc_pyramid_rgba16f(uint16_t* chain, uint32_t w, uint32_t h, uint32_t mipLevels, const orc_step* steps,
+                        uint32_t nSteps, int packedFp16)
+{
+  if(mipLevels == 0)
+    mipLevels = orc_full_levels(w, h);
+  if(check_plan(w, h, mipLevels, steps, nSteps))
+    return -1;
+  uint64_t offs[32];
+  orc_packed_offsets(w, h, mipLevels, offs);
+  for(uint32_t s = 0; s < nSteps; ++s)
+  {
+    uint32_t L = steps[s].inputLevel, count = steps[s].levelCount;
+    uint32_t sw, sh;
+    orc_level_dims(w, h, L, &sw, &sh);
+    const uint64_t n0 = (uint64_t)sw * sh;
+    f4* prevf         = packedFp16 ? NULL : (f4*)malloc(sizeof(f4) * n0);
+    h4* prevh         = packedFp16 ? (h4*)malloc(sizeof(h4) * n0) : NULL;
+    const uint16_t* in = chain + offs[L] * 4u;
+    for(uint64_t i = 0; i < n0; ++i)
+      for(int c = 0; c < 4; ++c)
+      {
+        if(packedFp16)
+          prevh[i].c[c] = orc_half_to_double(in[4 * i + c]);
+        else
+          prevf[i].c[c] = orc_half_to_float(in[4 * i + c]);
+      }
+    for(uint32_t r = 1; r <= count; ++r)
+    {
+      uint32_t dw, dh;
+      orc_level_dims(w, h, L + r, &dw, &dh);
+      const uint64_t n1 = (uint64_t)dw * dh;
+      uint16_t* out     = chain + offs[L + r] * 4u;
+      const int kx = ksize(sw), ky = ksize(sh);
+      if(packedFp16)
+      {
+        h4* cur = (h4*)malloc(sizeof(h4) * n1);
+        for(uint32_t y = 0; y < dh; ++y)
+          for(uint32_t x = 0; x < dw; ++x)
+          {
+            h4 v;
+            if(steps[s].pipeline == ORC_PIPE_FAST)
+            {
+              const h4* p = prevh + (uint64_t)(2u * y) * sw + 2u * x;
+              v           = box_h16(p[0], p[1], p[sw], p[sw + 1], fast_order(count, r));
+            }
+            else
+              v = general_h16(prevh, sw, dw, dh, x, y, kx, ky);
+            cur[(uint64_t)y * dw + x] = v;
+            for(int c = 0; c < 4; ++c)
+              out[4 * ((uint64_t)y * dw + x) + c] = orc_half_from_double(v.c[c]);
+          }
+        free(prevh);
+        prevh = cur;
+      }
+      else
+      {
+        f4* cur = (f4*)malloc(sizeof(f4) * n1);
+        for(uint32_t y = 0; y < dh; ++y)
+          for(uint32_t x = 0; x < dw; ++x)
+          {
+            f4 v;
+            if(steps[s].pipeline == ORC_PIPE_FAST)
+            {
+              const f4* p = prevf + (uint64_t)(2u * y) * sw + 2u * x;
+              v           = box_f32(p[0], p[1], p[sw], p[sw + 1], fast_order(count, r));
+            }
+            else
+              v = general_f32(prevf, sw, dw, dh, x, y, kx, ky);
+            cur[(uint64_t)y * dw + x] = v;
+            for(int c = 0; c < 4; ++c)
+              out[4 * ((uint64_t)y * dw + x) + c] = orc_half_from_double((double)v.c[c]);
+          }
+        free(prevf);
+        prevf = cur;
+      }
+      sw = dw;
+      sh = dh;
+    }
+    free(prevf);
+    free(prevh);
+  }
+  return 0;
+}

@@ -97,6 +97,16 @@ uint32_t orc_compare_srgba8(const uint8_t* a, const uint8_t* b, uint32_t w, uint
 int orc_pyramid_rgba32f_f32(float* chain, uint32_t w, uint32_t h, uint32_t mipLevels,
                             const orc_step* steps, uint32_t nSteps);
 
+/* binary16 conversions (round to nearest even, subnormals, overflow to inf). */
+double orc_half_to_double(uint16_t h);
+float orc_half_to_float(uint16_t h);
+uint16_t orc_half_from_double(double x);
+/* RGBA16F chains: FP32 arithmetic (packedFp16 = 0; loads widen, stores round)
+ * or PACKED_FP16 arithmetic (2-tap / 2x2 stages in binary16, 3-tap stages in
+ * FP32 then rounded to binary16). */
+int orc_pyramid_rgba16f(uint16_t* chain, uint32_t w, uint32_t h, uint32_t mipLevels, const orc_step* steps,
+                        uint32_t nSteps, int packedFp16);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,12 @@ def lib():
         L.orc_pyramid_rgba32f_f32.restype = ctypes.c_int
         L.orc_pyramid_rgba32f_f32.argtypes = [p, u32, u32, u32, p, u32]
         L.orc_pyramid_linear_f64.argtypes = [p, u32, u32, u32]
+        L.orc_half_to_double.restype = ctypes.c_double
+        L.orc_half_to_double.argtypes = [ctypes.c_uint16]
+        L.orc_half_from_double.restype = ctypes.c_uint16
+        L.orc_half_from_double.argtypes = [ctypes.c_double]
+        L.orc_pyramid_rgba16f.restype = ctypes.c_int
+        L.orc_pyramid_rgba16f.argtypes = [p, u32, u32, u32, p, u32, ctypes.c_int]
         L.orc_reference_cpu_srgba8.argtypes = [p, u32, u32]
         L.orc_reference_cpu_srgba8_mt.argtypes = [p, u32, u32, u32]
         L.orc_compare_srgba8.restype = u32
@@ -132,6 +138,17 @@ def pyramid_rgba32f(chain: np.ndarray, w, h, steps, mip_levels=0) -> np.ndarray:
     arr = _steps_array(steps)
     rc = lib().orc_pyramid_rgba32f_f32(_ptr(out), w, h, mip_levels, ctypes.cast(arr, ctypes.c_void_p),
                                        len(steps))
+    if rc != 0:
+        raise ValueError("emulator rejected the plan")
+    return out
+
+
+def pyramid_rgba16f(chain: np.ndarray, w, h, steps, mip_levels=0, packed=False) -> np.ndarray:
+    """RGBA16F emulator on a packed chain of uint16 half bits; returns it."""
+    out = np.ascontiguousarray(chain).view(np.uint16).copy()
+    arr = _steps_array(steps)
+    rc = lib().orc_pyramid_rgba16f(_ptr(out), w, h, mip_levels, ctypes.cast(arr, ctypes.c_void_p), len(steps),
+                                   1 if packed else 0)
     if rc != 0:
         raise ValueError("emulator rejected the plan")
     return out

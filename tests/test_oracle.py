@@ -451,3 +451,48 @@ def test_mixed_sizes_distribution():
     assert odd + even == 512
     assert 0.3 < odd / 512 < 0.5
     assert all(64 <= a <= 4096 and 64 <= b <= 4096 for a, b in sizes)
+
+
+# ---------------------------------------------------------------------------
+# binary16 (RGBA16F)
+# ---------------------------------------------------------------------------
+
+def test_half_conversions_match_numpy(orc):
+    """numpy's float64 -> float16 cast rounds to nearest even with subnormals."""
+    L = orc.lib()
+    rng = np.random.default_rng(16)
+    xs = np.concatenate([rng.standard_normal(2000) * 10.0 ** rng.integers(-9, 6, 2000),
+                         [0.0, -0.0, 65504.0, 65519.99, 65520.0, 1e9, 2.0 ** -24, 2.0 ** -25, 3 * 2.0 ** -26,
+                          2.0 ** -14, 2.0 ** -14 - 2.0 ** -25, 1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11]])
+    for x in xs:
+        want = np.array([x], np.float64).astype(np.float16).view(np.uint16)[0]
+        got = L.orc_half_from_double(float(x))
+        assert got == want, (x, hex(got), hex(want))
+    for b in range(0, 65536, 7):
+        h = np.array([b], np.uint16).view(np.float16)[0]
+        if np.isfinite(h):
+            assert L.orc_half_to_double(b) == float(h)
+
+
+@pytest.mark.parametrize("w,h", [(256, 256), (257, 129), (1920, 1080)])
+def test_rgba16f_emulators_track_fp64(w, h, orc):
+    """Both RGBA16F arithmetics stay close to the FP64 filter: FP32 within a few
+    half ulps, packed binary16 within a few more (errors compound per level)."""
+    l0 = synth.rgba16f(w, h, 7)
+    offs, total = P.packed_layout(w, h, 0, 8)
+    chain = np.zeros(total // 2, np.uint16)
+    chain[: w * h * 4] = l0.view(np.uint16).reshape(-1)
+    steps = orc.plan(w, h)
+    f32 = orc.pyramid_rgba16f(chain, w, h, steps, 0, packed=False).view(np.float16)
+    pk = orc.pyramid_rgba16f(chain, w, h, steps, 0, packed=True).view(np.float16)
+    lin = orc.pyramid_linear_f64(linear_chain(w, h, l0.astype(np.float64)), w, h)
+    toffs, _ = P.packed_layout(w, h, 0, 1)
+    for l in range(1, P.full_level_count(w, h)):
+        lw, lh = P.level_dims(w, h, l)
+        ref = lin[toffs[l]: toffs[l] + lw * lh].reshape(-1)
+        a = f32[4 * toffs[l]: 4 * (toffs[l] + lw * lh)].astype(np.float64)
+        b = pk[4 * toffs[l]: 4 * (toffs[l] + lw * lh)].astype(np.float64)
+        rel_a = np.max(np.abs(a - ref) / np.abs(ref))
+        rel_b = np.max(np.abs(b - ref) / np.abs(ref))
+        assert rel_a < 2e-3, (l, rel_a)  # one half-ulp rounding (4.9e-4) per stored level at most 2 deep
+        assert rel_b < 2e-2, (l, rel_b)

@@ -6,16 +6,19 @@
 // is a struct of static device functions; the kernels are templates over it.
 //
 //   Raw                  storage texel (what a level holds in memory)
+//   Val                  NVPRO_PYRAMID_TYPE: the value reductions operate on
 //   Tables               per-workgroup LDS tables (may be empty)
 //   fill<BS>(t, scale)   workgroup-cooperative set-up by BS threads; `scale`
 //                        premultiplies decoded values (0.25 for the 2x2 box)
-//   load<_, PRE>(t, raw) NVPRO_PYRAMID_LOAD: Raw -> float4 linear; table-driven
-//                        formats fold PRE into the fill scale, others multiply
-//   store(t, v)          NVPRO_PYRAMID_STORE: float4 linear -> Raw
+//   load<PRE>(t, raw)    NVPRO_PYRAMID_LOAD: Raw -> Val (times 0.25 if PRE)
+//   store(t, v)          NVPRO_PYRAMID_STORE: Val -> Raw
+//   add, mul, reduce3    the arithmetic REDUCE, REDUCE2 and REDUCE4 are built
+//                        from (box, reduce2, reduce3 below)
+//   kPrescale            whether load<true> then box<true> equals box<false>
+//                        of plain loads bit for bit (exact scaling by 0.25)
 //
-// The arithmetic on float4 (REDUCE, REDUCE2, REDUCE4) is shared:
-// reduce3, reduce2 and box.  Every FP32 operation is written out; the library is
-// compiled with -ffp-contract=off so none is fused behind our back.
+// Every FP32 operation is written out; the library is compiled with
+// -ffp-contract=off so none is fused behind our back.
 #pragma once
 
 #include <stdint.h>
@@ -41,31 +44,50 @@ __device__ __forceinline__ float4 f4mul(float4 a, float s)
 
 // NVPRO_PYRAMID_REDUCE of srgba8_mipmap_preamble.glsl:24-25, a0*v0 + a1*v1 + a2*v2,
 // with the contraction fixed as fma(a2, v2, fma(a1, v1, a0 * v0)).
-__device__ __forceinline__ float reduce3(float w0, float v0, float w1, float v1, float w2, float v2)
+__device__ __forceinline__ float fma3(float w0, float v0, float w1, float v1, float w2, float v2)
 {
   return __builtin_fmaf(w2, v2, __builtin_fmaf(w1, v1, w0 * v0));
 }
-__device__ __forceinline__ float4 reduce3(float w0, float4 v0, float w1, float4 v1, float w2, float4 v2)
+__device__ __forceinline__ float4 fma3(float w0, float4 v0, float w1, float4 v1, float w2, float4 v2)
 {
-  return make_float4(reduce3(w0, v0.x, w1, v1.x, w2, v2.x), reduce3(w0, v0.y, w1, v1.y, w2, v2.y),
-                     reduce3(w0, v0.z, w1, v1.z, w2, v2.z), reduce3(w0, v0.w, w1, v1.w, w2, v2.w));
+  return make_float4(fma3(w0, v0.x, w1, v1.x, w2, v2.x), fma3(w0, v0.y, w1, v1.y, w2, v2.y),
+                     fma3(w0, v0.z, w1, v1.z, w2, v2.z), fma3(w0, v0.w, w1, v1.w, w2, v2.w));
 }
 
+// FP32 arithmetic on float4 values, shared by the FP32 functors.
+struct Fp32Arith
+{
+  using Val                          = float4;
+  static constexpr bool kPrescale    = true;
+  __device__ static Val add(Val a, Val b) { return f4add(a, b); }
+  __device__ static Val mul(Val a, float s) { return f4mul(a, s); }
+  __device__ static Val reduce3(float w0, Val v0, float w1, Val v1, float w2, Val v2)
+  {
+    return fma3(w0, v0, w1, v1, w2, v2);
+  }
+};
+
 // NVPRO_PYRAMID_REDUCE2 of srgba8_mipmap_preamble.glsl:35: 0.5 * (v0 + v1).
-__device__ __forceinline__ float4 reduce2(float4 v0, float4 v1) { return f4mul(f4add(v0, v1), 0.5f); }
+template <class F>
+__device__ __forceinline__ typename F::Val reduce2(typename F::Val v0, typename F::Val v1)
+{
+  return F::mul(F::add(v0, v1), 0.5f);
+}
 
 // NVPRO_PYRAMID_REDUCE4 of srgba8_mipmap_preamble.glsl:37-38,
 // 0.25 * ((v00 + v01) + (v10 + v11)).  The reference passes the four quadrants
 // of a 2x2 square in one of two orders (nvpro_pyramid.glsl:179-188 vs
 // :364-392); `order` selects which pairs are added first:
 //   order 0: (UL + LL) + (UR + LR)      order 1: (UL + UR) + (LL + LR)
-// PRE: the inputs were already multiplied by 0.25 (exact), so the sum is the
+// PRE: the inputs were already multiplied by 0.25 (exactly), so the sum is the
 // result.
-template <bool PRE>
-__device__ __forceinline__ float4 box(float4 ul, float4 ur, float4 ll, float4 lr, int order)
+template <class F, bool PRE>
+__device__ __forceinline__ typename F::Val box(typename F::Val ul, typename F::Val ur, typename F::Val ll,
+                                               typename F::Val lr, int order)
 {
-  float4 s = order == 0 ? f4add(f4add(ul, ll), f4add(ur, lr)) : f4add(f4add(ul, ur), f4add(ll, lr));
-  return PRE ? s : f4mul(s, 0.25f);
+  const typename F::Val s =
+      order == 0 ? F::add(F::add(ul, ll), F::add(ur, lr)) : F::add(F::add(ul, ur), F::add(ll, lr));
+  return PRE ? s : F::mul(s, 0.25f);
 }
 
 // Summation order of relative level r of a fast dispatch filling `count` levels
@@ -85,7 +107,7 @@ __host__ __device__ __forceinline__ int fastOrder(uint32_t count, uint32_t r)
 // ---------------------------------------------------------------------------
 // sRGBA8: srgba8_mipmap_preamble.glsl:19-128 with exact conversions.
 // ---------------------------------------------------------------------------
-struct Srgba8
+struct Srgba8 : Fp32Arith
 {
   using Raw                        = uint32_t;
   static constexpr uint32_t kBytes = 4;
@@ -169,7 +191,7 @@ struct Srgba8
 // ---------------------------------------------------------------------------
 // RGBA32F: linear float4 texels, identity load and store.
 // ---------------------------------------------------------------------------
-struct Rgba32f
+struct Rgba32f : Fp32Arith
 {
   struct __attribute__((aligned(4))) Raw
   {
@@ -199,7 +221,7 @@ struct Rgba32f
 // ---------------------------------------------------------------------------
 // RGBA16F: linear half4 texels; FP32 arithmetic, round-to-nearest-even on store.
 // ---------------------------------------------------------------------------
-struct Rgba16f
+struct Rgba16f : Fp32Arith
 {
   struct __attribute__((aligned(4))) Raw
   {
@@ -225,6 +247,53 @@ struct Rgba16f
   __device__ static Raw store(const Tables&, float4 v)
   {
     return Raw{f2h(v.x) | f2h(v.y) << 16, f2h(v.z) | f2h(v.w) << 16};
+  }
+};
+
+// ---------------------------------------------------------------------------
+// RGBA16F with PACKED_FP16 arithmetic: 2x2 boxes and 2-tap reductions run in
+// binary16 on (r,g) and (b,a) pairs (v_pk_add_f16 / v_pk_mul_f16); 3-tap
+// reductions widen to FP32 for the weights and round the result to binary16.
+// ---------------------------------------------------------------------------
+struct Rgba16fPacked
+{
+  using Raw                        = Rgba16f::Raw;
+  static constexpr uint32_t kBytes = 8;
+  // Scaling a binary16 subnormal by 0.25 can round, so no premultiplied loads.
+  static constexpr bool kPrescale = false;
+  struct Val
+  {
+    __half2 rg, ba;
+  };
+  using Tables = Rgba16f::Tables;
+  template <int BS>
+  __device__ static void fill(Tables&, float)
+  {
+  }
+  template <bool PRE>
+  __device__ static Val load(const Tables&, Raw t)
+  {
+    static_assert(!PRE, "binary16 values are not premultiplied");
+    return Val{__builtin_bit_cast(__half2, t.lo), __builtin_bit_cast(__half2, t.hi)};
+  }
+  __device__ static Raw store(const Tables&, Val v)
+  {
+    return Raw{__builtin_bit_cast(uint32_t, v.rg), __builtin_bit_cast(uint32_t, v.ba)};
+  }
+  __device__ static Val add(Val a, Val b) { return Val{__hadd2(a.rg, b.rg), __hadd2(a.ba, b.ba)}; }
+  __device__ static Val mul(Val a, float s)
+  {
+    const __half2 k = __float2half2_rn(s);  // 0.25 and 0.5 are exact in binary16
+    return Val{__hmul2(a.rg, k), __hmul2(a.ba, k)};
+  }
+  __device__ static Val reduce3(float w0, Val v0, float w1, Val v1, float w2, Val v2)
+  {
+    const float2 a0 = __half22float2(v0.rg), a1 = __half22float2(v1.rg), a2 = __half22float2(v2.rg);
+    const float2 b0 = __half22float2(v0.ba), b1 = __half22float2(v1.ba), b2 = __half22float2(v2.ba);
+    return Val{__halves2half2(__float2half_rn(fma3(w0, a0.x, w1, a1.x, w2, a2.x)),
+                              __float2half_rn(fma3(w0, a0.y, w1, a1.y, w2, a2.y))),
+               __halves2half2(__float2half_rn(fma3(w0, b0.x, w1, b1.x, w2, b2.x)),
+                              __float2half_rn(fma3(w0, b0.y, w1, b1.y, w2, b2.y)))};
   }
 };
 

@@ -115,6 +115,22 @@ __device__ __forceinline__ uint32_t compactEven(uint32_t v)
   return v;
 }
 
+// Segment of a batched launch that holds group g (segments sorted by groupStart;
+// g is uniform, so this binary search runs on scalar loads).
+__device__ __forceinline__ uint32_t findSegment(const BatchSeg* segs, uint32_t n, uint32_t g)
+{
+  uint32_t lo = 0, hi = n;
+  while(hi - lo > 1u)
+  {
+    const uint32_t mid = (lo + hi) >> 1;
+    if(segs[mid].groupStart <= g)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 // kernelSizeFromInputSize_ (nvpro_pyramid.glsl:557-561).
 __host__ __device__ __forceinline__ int kernelSize(uint32_t dim) { return dim == 1u ? 1 : (int)(2u | (dim & 1u)); }
 
@@ -136,21 +152,22 @@ __device__ __forceinline__ Weights3 weights3(uint32_t n, uint32_t d)
 // reduceStoreSample_ (nvpro_pyramid.glsl:575-656) without the store: the sample
 // of destination texel (dx, dy) of a (dw x dh) level from a kx x ky window whose
 // texel at (column j, row i) `at(j, i)` returns.  Vertical first, then horizontal.
-template <class At>
-__device__ __forceinline__ float4 generalSample(At at, int kx, int ky, uint32_t dw, uint32_t dh, uint32_t dx,
-                                                uint32_t dy)
+template <class F, class At>
+__device__ __forceinline__ typename F::Val generalSample(At at, int kx, int ky, uint32_t dw, uint32_t dh, uint32_t dx,
+                                                         uint32_t dy)
 {
+  using V           = typename F::Val;
   const Weights3 wy = weights3(dh, dy);
-  float4 h[3];
+  V h[3];
 #pragma unroll
   for(int j = 0; j < 3; ++j)
   {
     if(j < kx)
     {
       if(ky == 3)
-        h[j] = reduce3(wy.w0, at(j, 0), wy.w1, at(j, 1), wy.w2, at(j, 2));
+        h[j] = F::reduce3(wy.w0, at(j, 0), wy.w1, at(j, 1), wy.w2, at(j, 2));
       else if(ky == 2)
-        h[j] = reduce2(at(j, 0), at(j, 1));
+        h[j] = reduce2<F>(at(j, 0), at(j, 1));
       else
         h[j] = at(j, 0);
     }
@@ -158,54 +175,64 @@ __device__ __forceinline__ float4 generalSample(At at, int kx, int ky, uint32_t 
   if(kx == 3)
   {
     const Weights3 wx = weights3(dw, dx);
-    return reduce3(wx.w0, h[0], wx.w1, h[1], wx.w2, h[2]);
+    return F::reduce3(wx.w0, h[0], wx.w1, h[1], wx.w2, h[2]);
   }
   if(kx == 2)
-    return reduce2(h[0], h[1]);
+    return reduce2<F>(h[0], h[1]);
   return h[0];
 }
 
-// DPP row shift-left: lane i reads lane i + n of its 16-lane row (bound_ctrl: 0
+// Cross-lane moves of any value made of 32-bit words.
+// DPP row shift-left: lane i reads lane i + N of its 16-lane row (bound_ctrl: 0
 // beyond the row).  Only the lowest lane of each reduced group is consumed.
-template <int N>
-__device__ __forceinline__ float dppShl(float v)
+template <int N, class V>
+__device__ __forceinline__ V dppShl(V v)
 {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x100 + N, 0xf, 0xf,
-                                                                true));
-}
-template <int N>
-__device__ __forceinline__ float4 dppShl(float4 v)
-{
-  return make_float4(dppShl<N>(v.x), dppShl<N>(v.y), dppShl<N>(v.z), dppShl<N>(v.w));
+  static_assert(sizeof(V) % 4 == 0, "32-bit words");
+  uint32_t w[sizeof(V) / 4];
+  __builtin_memcpy(w, &v, sizeof v);
+#pragma unroll
+  for(int k = 0; k < (int)(sizeof(V) / 4); ++k)
+    w[k] = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)w[k], 0x100 + N, 0xf, 0xf, true);
+  V r;
+  __builtin_memcpy(&r, w, sizeof r);
+  return r;
 }
 
-__device__ __forceinline__ float4 shflXor(float4 v, int mask)
+template <class V>
+__device__ __forceinline__ V shflXor(V v, int mask)
 {
-  return make_float4(__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64), __shfl_xor(v.z, mask, 64),
-                     __shfl_xor(v.w, mask, 64));
+  uint32_t w[sizeof(V) / 4];
+  __builtin_memcpy(w, &v, sizeof v);
+#pragma unroll
+  for(int k = 0; k < (int)(sizeof(V) / 4); ++k)
+    w[k] = (uint32_t)__shfl_xor((int)w[k], mask, 64);
+  V r;
+  __builtin_memcpy(&r, w, sizeof r);
+  return r;
 }
 
 // One cross-lane 2x2 reduction level: the group leader (lowest lane) combines its
 // value with lanes +mx (x neighbour), +my (y neighbour) and +mx+my.
-template <int MX>
-__device__ __forceinline__ float4 crossLaneBox(float4 v, int order)
+template <class F, int MX>
+__device__ __forceinline__ typename F::Val crossLaneBox(typename F::Val v, int order)
 {
   constexpr int MY = 2 * MX;
   if constexpr(MY < 16)
   {
     if(order == 0)
     {
-      float4 s = f4add(v, dppShl<MY>(v));
-      return f4mul(f4add(s, dppShl<MX>(s)), 0.25f);
+      const typename F::Val s = F::add(v, dppShl<MY>(v));
+      return F::mul(F::add(s, dppShl<MX>(s)), 0.25f);
     }
-    float4 s = f4add(v, dppShl<MX>(v));
-    return f4mul(f4add(s, dppShl<MY>(s)), 0.25f);
+    const typename F::Val s = F::add(v, dppShl<MX>(v));
+    return F::mul(F::add(s, dppShl<MY>(s)), 0.25f);
   }
   else
   {
     const int first = order == 0 ? MY : MX, second = order == 0 ? MX : MY;
-    float4 s        = f4add(v, shflXor(v, first));
-    return f4mul(f4add(s, shflXor(s, second)), 0.25f);
+    const typename F::Val s = F::add(v, shflXor(v, first));
+    return F::mul(F::add(s, shflXor(s, second)), 0.25f);
   }
 }
 

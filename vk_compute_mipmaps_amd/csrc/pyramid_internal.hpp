@@ -36,8 +36,15 @@ struct TailStep
 };
 struct TailPlan
 {
-  uint32_t count, pad[3];
+  uint32_t count, image, pad[2];  // image: index into the batch's ImageDesc array
   TailStep steps[NVPRO_PYRAMID_MAX_LEVELS];
+};
+
+// One image's share of a batched launch: tile groups
+// [groupStart, next segment's groupStart) belong to step (inputLevel) of image.
+struct BatchSeg
+{
+  uint32_t image, inputLevel, tilesX, numTiles, groupStart, pad[3];
 };
 
 // A step whose first output level has at most kTailMaxTexels texels, and every
@@ -50,6 +57,19 @@ hipError_t launchFast(hipStream_t stream, const ImageDesc& img, uint32_t inputLe
                       uint32_t wavesPerCU);
 hipError_t launchGeneral(hipStream_t stream, const ImageDesc& img, uint32_t inputLevel, uint32_t levelCount);
 hipError_t launchTail(hipStream_t stream, const ImageDesc& img, const TailPlan& plan);
+
+// Batched launchers: one launch covers the segments of many images (device
+// arrays).  Fast: all segments fill `count` levels.  General: `count` 1 or 2.
+hipError_t launchFastBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic, uint32_t count,
+                           const ImageDesc* imgs, const BatchSeg* segs, uint32_t numSegs, uint32_t totalGroups,
+                           uint32_t wgPerCU);
+hipError_t launchGeneralBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic, uint32_t count,
+                              const ImageDesc* imgs, const BatchSeg* segs, uint32_t numSegs, uint32_t totalGroups);
+hipError_t launchTailBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic, const ImageDesc* imgs,
+                           const TailPlan* plans, uint32_t numPlans);
+// Tile groups one launch of the fast / general kernel uses for a step.
+uint32_t fastGroupsPerStep(uint32_t width, uint32_t height, uint32_t count);
+uint32_t generalGroupsPerStep(uint32_t dstWidth, uint32_t dstHeight, uint32_t count, uint32_t* tilesX);
 
 // Launch a validated plan: per-step kernels for the large steps, then one tail
 // launch for the small remainder.
