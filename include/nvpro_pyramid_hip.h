@@ -173,6 +173,82 @@ uint32_t nvproPyramidBytesPerTexel(uint32_t format);
 /* Message for the last failed call on this thread ("" if none). */
 const char* nvproPyramidGetLastError(void);
 
+/* Workload generator (no reference counterpart; the reference's benchmark loads
+ * image files instead, demo_app/mipmaps_app.cpp:556-879): writes the seeded
+ * synthetic level 0 of `image` on `stream` (SRGBA8: smooth value noise with a
+ * 64-px period plus uniform noise, uniform alpha; RGBA16F: log-uniform radiance
+ * in [1e-3, 1e3), uniform alpha), bit-identical to vk_compute_mipmaps_amd/synth.py. */
+int nvproPyramidSynthesize(hipStream_t stream, uint32_t format, const NvproPyramidImage* image, uint64_t seed);
+
+/* ------------------------------------------------------------------------- */
+/* Batched chains.  No reference counterpart: the reference loops over images  */
+/* and calls nvproCmdPyramidDispatch once per image (demo_app/mipmaps_app.cpp: */
+/* 672-722).  A batch runs the same per-image plans (default dispatchers), so   */
+/* every image's levels are identical to what nvproPyramidDispatch writes, but  */
+/* the steps of all images go through one persistent work-queue launch (QUEUE)  */
+/* or one launch per round of independent steps (ROUNDS).                       */
+/* ------------------------------------------------------------------------- */
+
+#define NVPRO_PYRAMID_BATCH_MODE_ROUNDS 0u
+#define NVPRO_PYRAMID_BATCH_MODE_QUEUE 1u
+
+/* Error flags reported by nvproPyramidBatchReadErrors. */
+#define NVPRO_PYRAMID_BATCH_ERROR_SPIN 1u /* a queue dependency wait gave up (results invalid) */
+
+typedef struct NvproPyramidBatchConfig {
+  uint32_t mode;            /* NVPRO_PYRAMID_BATCH_MODE_* */
+  uint32_t workgroupsPerCU; /* persistent workgroups per compute unit; 0: as many as fit */
+  uint32_t itemBytes;       /* input bytes per work item (a run of groups of one step); 0: 65536 */
+  uint32_t tailTexels;      /* a step whose first output level has at most this many texels, and
+                               the rest of its image's plan, run in one workgroup; 0: never;
+                               at most 128 */
+  uint32_t useGraph;        /* nonzero: nvproPyramidBatchLaunch replays a hipGraph built at creation */
+  uint32_t reserved[3];     /* must be 0 */
+} NvproPyramidBatchConfig;
+
+typedef struct NvproPyramidBatchInfo {
+  uint32_t imageCount, segmentCount, itemCount, roundCount;
+  uint32_t launchCount;      /* kernel launches per nvproPyramidBatchLaunch */
+  uint32_t workgroups;       /* workgroups of the largest launch */
+  uint64_t algorithmicBytes; /* every base level read once + every output level written once */
+  uint64_t deviceBytes;      /* the batch's device work-list size */
+  NvproPyramidBatchConfig config;
+} NvproPyramidBatchInfo;
+
+typedef struct NvproPyramidBatch NvproPyramidBatch;
+
+/* Defaults: QUEUE, as many workgroups as fit, 64 KiB items, 128-texel tails, graph on. */
+int nvproPyramidBatchDefaultConfig(NvproPyramidBatchConfig* config);
+
+/* Plans `imageCount` images of one format (mipLevels: per-image level counts,
+ * NULL or 0 entries for full chains), allocates and uploads the work list
+ * (synchronously, on a private stream) and, with useGraph, builds the graph.
+ * config NULL: defaults.  The images must stay allocated while the batch is
+ * launched; their contents may change between launches. */
+int nvproPyramidBatchCreate(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* images,
+                            const uint32_t* mipLevels, uint32_t imageCount,
+                            const NvproPyramidBatchConfig* config, NvproPyramidBatch** batch);
+
+/* Enqueues the whole batch on `stream`.  Launches of one batch object must not
+ * overlap in time (they share its counters): order them on one stream. */
+int nvproPyramidBatchLaunch(NvproPyramidBatch* batch, hipStream_t stream);
+
+int nvproPyramidBatchGetInfo(const NvproPyramidBatch* batch, NvproPyramidBatchInfo* info);
+
+/* Synchronises with the device, returns the batch's sticky error flags and clears them. */
+int nvproPyramidBatchReadErrors(NvproPyramidBatch* batch, uint32_t* flags);
+
+int nvproPyramidBatchDestroy(NvproPyramidBatch* batch);
+
+/* Project autotuner: times each candidate configuration (mode, workgroups per
+ * CU, item size, tail size; graph on) over `repetitions` launches of these
+ * images on `stream` and returns the fastest.  It writes the images' output
+ * levels (with the same values every time).  Results are cached per device,
+ * format and batch shape for the life of the process. */
+int nvproPyramidBatchAutotune(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* images,
+                              const uint32_t* mipLevels, uint32_t imageCount, hipStream_t stream,
+                              uint32_t repetitions, NvproPyramidBatchConfig* best, float* bestMicroseconds);
+
 #ifdef __cplusplus
 }
 #endif

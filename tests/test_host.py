@@ -144,3 +144,49 @@ def test_null_dispatcher_records_nothing():
 def test_plan_reports_reference_workgroups_and_push_constants():
     steps = P.plan(4096, 4096)
     assert [(s.pushConstant, s.groupCountX, s.inputWidth) for s in steps] == [(6, 4096, 4096), ((6 << 5) | 6, 1, 64)]
+
+
+def test_batch_argument_errors_before_any_device_call():
+    """Batch creation validates config, pipelines and every image before touching the device."""
+    fake = [P.make_image(0x10000, 64, 64), P.make_image(0x20000, 33, 7)]
+    pipes = P.pipelines()
+    assert "tailTexels" in _err(P.PyramidBatch, pipes, fake, None, P.batch_config(tail_texels=129))
+    cfg = P.batch_config()
+    cfg.reserved[1] = 5
+    assert "reserved" in _err(P.PyramidBatch, pipes, fake, None, cfg)
+    cfg = P.batch_config()
+    cfg.mode = 7
+    assert "unknown mode" in _err(P.PyramidBatch, pipes, fake, None, cfg)
+    assert "no images" in _err(P.PyramidBatch, pipes, [], None, None)
+    assert "unknown format" in _err(P.PyramidBatch, P.NvproPyramidPipelines(9, 1, 0, 0), fake, None, None)
+    bad = [fake[0], P.make_image(0x30000, 0, 5)]
+    assert "zero image size" in _err(P.PyramidBatch, pipes, bad, None, None)
+    assert "exceeds" in _err(P.PyramidBatch, pipes, fake, [0, 7], None)
+    assert "no images" in _err(P.batch_autotune, pipes, [])
+
+
+def test_batch_default_config():
+    c = P.batch_config()
+    d = P.NvproPyramidBatchConfig()
+    assert P.lib().nvproPyramidBatchDefaultConfig(ctypes.byref(d)) == 0
+    assert d.as_dict() == {"mode": "queue", "workgroupsPerCU": 0, "itemBytes": 65536, "tailTexels": 128,
+                           "useGraph": True}
+    assert c.as_dict() == d.as_dict()
+
+
+def test_synthesize_argument_errors():
+    img = P.make_image(0x10000, 8, 8)
+    assert "no generator" in _err(P.synthesize, 0, P.FORMAT_RGBA32F, img, 1)
+    assert "zero image size" in _err(P.synthesize, 0, P.FORMAT_SRGBA8, P.make_image(0x10000, 0, 8), 1)
+    assert "aligned" in _err(P.synthesize, 0, P.FORMAT_RGBA16F, P.make_image(0x10004, 8, 8), 1)
+
+
+def test_demo_workload_sizes_match_python():
+    """The C++ host's mixed-size workload equals synth.mixed_sizes (no device use)."""
+    import subprocess
+
+    from vk_compute_mipmaps_amd import build, synth
+
+    out = subprocess.run([build.DEMO_PATH, "sizes", "512", "8"], check=True, capture_output=True, text=True).stdout
+    got = [tuple(int(v) for v in line.split()) for line in out.splitlines()]
+    assert got == synth.mixed_sizes(512, 8)

@@ -20,11 +20,16 @@ LIB_DIR = os.path.join(PKG_DIR, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libnvpro_pyramid_hip.so")
 INCLUDE = os.path.join(REPO_DIR, "include")
 
+DEMO_PATH = os.path.join(LIB_DIR, "nvpro_pyramid_demo")
+
 SOURCES = [
     "fast_pipeline.hip",
     "general_pipeline.hip",
     "tail_pipeline.hip",
+    "batch_pipeline.hip",
+    "synth.hip",
     "nvpro_pyramid_dispatch.cpp",
+    "nvpro_pyramid_batch.cpp",
 ]
 
 HIPCC_FLAGS = [
@@ -65,7 +70,21 @@ def build_library(verbose: bool = True) -> str:
     os.replace(tmp, LIB_PATH)
     for obj in objs:
         os.remove(obj)
+    build_demo(verbose)
     return LIB_PATH
+
+
+def build_demo(verbose: bool = True) -> str:
+    """The C++ host (csrc/pyramid_demo.cpp) linked against the library."""
+    tmp = DEMO_PATH + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", "-I", INCLUDE,
+           os.path.join(CSRC, "pyramid_demo.cpp"), "-o", tmp, "-L", LIB_DIR, "-lnvpro_pyramid_hip",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, DEMO_PATH)
+    return DEMO_PATH
 
 
 if __name__ == "__main__":

@@ -10,6 +10,8 @@
 //   Tables               per-workgroup LDS tables (may be empty)
 //   fill<BS>(t, scale)   workgroup-cooperative set-up by BS threads; `scale`
 //                        premultiplies decoded values (0.25 for the 2x2 box)
+//   DualTables,          tables serving load<true> (times 0.25) and
+//   fillDual<BS>(t)      load<false> (times 1) side by side
 //   load<PRE>(t, raw)    NVPRO_PYRAMID_LOAD: Raw -> Val (times 0.25 if PRE)
 //   store(t, v)          NVPRO_PYRAMID_STORE: Val -> Raw
 //   add, mul, reduce3    the arithmetic REDUCE, REDUCE2 and REDUCE4 are built
@@ -47,6 +49,16 @@ __device__ __forceinline__ float4 f4mul(float4 a, float s)
 __device__ __forceinline__ float fma3(float w0, float v0, float w1, float v1, float w2, float v2)
 {
   return __builtin_fmaf(w2, v2, __builtin_fmaf(w1, v1, w0 * v0));
+}
+
+// Round an FP32 value to binary16 (nearest even) after it has been rounded to
+// FP32.  The empty asm keeps the compiler from merging the producing FMA and the
+// conversion into one mixed-precision instruction (v_fma_mixlo_f16), which
+// would round once instead of twice.
+__device__ __forceinline__ __half toHalf(float f)
+{
+  __asm__ volatile("" : "+v"(f));
+  return __float2half_rn(f);
 }
 __device__ __forceinline__ float4 fma3(float w0, float4 v0, float w1, float4 v1, float w2, float4 v2)
 {
@@ -160,13 +172,40 @@ struct Srgba8 : Fp32Arith
     }
   }
 
+  // Tables for kernels that run premultiplied (fast) and plain (general, tail)
+  // loads alike: dec/adec hold the decode times 0.25, dec1/adec1 times 1.
+  struct DualTables : Tables
+  {
+    float dec1[256];
+    float adec1[256];
+  };
+
+  template <int BS>
+  __device__ static void fillDual(DualTables& t)
+  {
+    fill<BS>(t, 0.25f);
+    for(uint32_t i = threadIdx.x; i < 256u; i += BS)
+    {
+      t.dec1[i]  = __uint_as_float(gSrgbDecode[i]);
+      t.adec1[i] = (float)i * (1.0f / 255.0f);
+    }
+  }
+
   // texelFetch of the sRGB sampler (preamble:21-22): sRGB decode of RGB and the
-  // alpha code * (1/255) (shaders/srgb.h:216), each a table lookup.
+  // alpha code * (1/255) (shaders/srgb.h:216), each a table lookup.  With plain
+  // Tables the fill scale applies whatever PRE says.
   template <bool PRE>
   __device__ static float4 load(const Tables& t, uint32_t texel)
   {
     return make_float4(t.dec[texel & 0xffu], t.dec[(texel >> 8) & 0xffu], t.dec[(texel >> 16) & 0xffu],
                        t.adec[texel >> 24]);
+  }
+  template <bool PRE>
+  __device__ static float4 load(const DualTables& t, uint32_t texel)
+  {
+    const float* d = PRE ? t.dec : t.dec1;
+    const float* a = PRE ? t.adec : t.adec1;
+    return make_float4(d[texel & 0xffu], d[(texel >> 8) & 0xffu], d[(texel >> 16) & 0xffu], a[texel >> 24]);
   }
 
   // srgbFromLinear (preamble:110-121), exact: the bucket of x's top bits gives the
@@ -202,8 +241,13 @@ struct Rgba32f : Fp32Arith
   {
     int unused;
   };
+  using DualTables = Tables;
   template <int BS>
   __device__ static void fill(Tables&, float)
+  {
+  }
+  template <int BS>
+  __device__ static void fillDual(Tables&)
   {
   }
   template <bool PRE>
@@ -232,12 +276,17 @@ struct Rgba16f : Fp32Arith
   {
     int unused;
   };
+  using DualTables = Tables;
   template <int BS>
   __device__ static void fill(Tables&, float)
   {
   }
+  template <int BS>
+  __device__ static void fillDual(Tables&)
+  {
+  }
   __device__ static float h2f(uint32_t bits) { return __half2float(__ushort_as_half((unsigned short)bits)); }
-  __device__ static uint32_t f2h(float f) { return __half_as_ushort(__float2half_rn(f)); }
+  __device__ static uint32_t f2h(float f) { return __half_as_ushort(toHalf(f)); }
   template <bool PRE>
   __device__ static float4 load(const Tables&, Raw t)
   {
@@ -265,9 +314,14 @@ struct Rgba16fPacked
   {
     __half2 rg, ba;
   };
-  using Tables = Rgba16f::Tables;
+  using Tables     = Rgba16f::Tables;
+  using DualTables = Tables;
   template <int BS>
   __device__ static void fill(Tables&, float)
+  {
+  }
+  template <int BS>
+  __device__ static void fillDual(Tables&)
   {
   }
   template <bool PRE>
@@ -290,10 +344,8 @@ struct Rgba16fPacked
   {
     const float2 a0 = __half22float2(v0.rg), a1 = __half22float2(v1.rg), a2 = __half22float2(v2.rg);
     const float2 b0 = __half22float2(v0.ba), b1 = __half22float2(v1.ba), b2 = __half22float2(v2.ba);
-    return Val{__halves2half2(__float2half_rn(fma3(w0, a0.x, w1, a1.x, w2, a2.x)),
-                              __float2half_rn(fma3(w0, a0.y, w1, a1.y, w2, a2.y))),
-               __halves2half2(__float2half_rn(fma3(w0, b0.x, w1, b1.x, w2, b2.x)),
-                              __float2half_rn(fma3(w0, b0.y, w1, b1.y, w2, b2.y)))};
+    return Val{__halves2half2(toHalf(fma3(w0, a0.x, w1, a1.x, w2, a2.x)), toHalf(fma3(w0, a0.y, w1, a1.y, w2, a2.y))),
+               __halves2half2(toHalf(fma3(w0, b0.x, w1, b1.x, w2, b2.x)), toHalf(fma3(w0, b0.y, w1, b1.y, w2, b2.y)))};
   }
 };
 

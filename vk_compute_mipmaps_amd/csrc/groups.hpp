@@ -1,0 +1,312 @@
+// Work-group bodies of the three pipelines, shared by the per-step kernels and
+// the batched kernel.  Each body does one unit of a step ("group"): a tile group
+// of a fast step, 256 outputs or an 8x8 tile of a general step, or a whole tail
+// plan.  They take the LDS decode/encode tables (`tab`, any table type the
+// functor's load/store accept) and the LDS scratch they need as pointers, so a
+// kernel can overlay the scratch of different bodies.  Every thread of the
+// 256-thread workgroup calls a body; bodies with barriers say so.
+#pragma once
+
+#include "kernel_common.hpp"
+
+namespace nvpro_pyramid {
+
+constexpr int kGroupBS = 256;  // threads per workgroup, every pipeline
+
+// ---------------------------------------------------------------------------
+// Fast pipeline (nvpro_pyramid.glsl:209-534).
+//
+// One step fills COUNT (1..6) levels from an input level whose edges are
+// divisible by 2^COUNT.  The input is cut into 2^COUNT x 2^COUNT tiles.  Each
+// lane owns a 4x4 input block (2x2 when COUNT is 1), loads it with one 16-byte
+// load per row and reduces it in registers to levels 1 and 2, so a group of 256
+// threads covers 4096 input texels: one 64x64 tile for COUNT 6, four 32x32
+// tiles for COUNT 5, and so on.  Lanes are numbered in Morton order within a
+// tile, so the level-2 values of a 2x2 are those of four consecutive threads.
+// Levels 3..COUNT are reduced by one "duty" wave per group (rotating over the
+// workgroup's waves with `iter`): the level-2 values go through a 4 KiB LDS
+// tile, duty lane k reduces threads 4k..4k+3 to level 3 (all 64 lanes busy), and
+// the levels above are cross-lane reductions inside the duty wave (DPP row
+// shifts, then ds_bpermute).  Per level the 2x2 summation order is the
+// reference's (fastOrder).
+// ---------------------------------------------------------------------------
+template <int COUNT>
+struct FastGeom
+{
+  static constexpr int LB           = COUNT >= 2 ? 2 : 1;                // lane block edge, log2
+  static constexpr int TEAM         = COUNT >= 2 ? 2 * (COUNT - 2) : 0;  // threads per tile, log2 (<= 8)
+  static constexpr int TILES_PER_WG = kGroupBS >> TEAM;
+  static constexpr bool UPPER       = COUNT >= 3;                        // levels 3.. exist
+};
+
+// Groups of a fast step over a width x height input level.
+__host__ __device__ inline uint32_t fastGroupCount(uint32_t width, uint32_t height, uint32_t count)
+{
+  const uint32_t tiles = (width >> count) * (height >> count);
+  const uint32_t team  = count >= 2u ? 2u * (count - 2u) : 0u;
+  const uint32_t perWG = (uint32_t)kGroupBS >> team;
+  return (tiles + perWG - 1u) / perWG;
+}
+
+// Group g of a fast step of `img` from level inLevel (tilesX x tilesY tiles in
+// all, numTiles = their product).  `level2`: kGroupBS values of LDS scratch
+// (unused when COUNT < 3).  Contains barriers when COUNT >= 3.
+template <class F, int COUNT, class TT>
+__device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2, const ImageDesc& img,
+                                          uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
+                                          uint32_t iter)
+{
+  using G            = FastGeom<COUNT>;
+  using Raw          = typename F::Raw;
+  using V            = typename F::Val;
+  constexpr int LB   = G::LB;
+  constexpr int BLK  = 1 << LB;
+  constexpr bool PRE = F::kPrescale;
+
+  const uint32_t lane   = threadIdx.x & 63u;
+  const uint32_t wave   = threadIdx.x >> 6;
+  const uint32_t inTile = threadIdx.x & ((1u << G::TEAM) - 1u);
+  const uint32_t px     = compactEven(inTile);
+  const uint32_t py     = compactEven(inTile >> 1);
+  const LevelDesc& in   = img.lv[inLevel];
+
+  // Levels 1 and 2 of this thread's block.
+  const uint32_t tile = g * G::TILES_PER_WG + (threadIdx.x >> G::TEAM);
+  V v{};
+  if(tile < numTiles)  // whole tiles
+  {
+    const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
+    const uint32_t x0 = (tx << COUNT) + (px << LB), y0 = (ty << COUNT) + (py << LB);
+    Raw raw[BLK][BLK];
+    const uint8_t* p = img.base + in.offset + (uint64_t)y0 * in.pitch + (uint64_t)x0 * F::kBytes;
+#pragma unroll
+    for(int r = 0; r < BLK; ++r)
+      loadRow<F, BLK>(p + (uint64_t)r * in.pitch, raw[r]);
+
+    // Level 1: LOAD_REDUCE4 as four loads (premultiplied by 0.25 when exact).
+    V l1[BLK / 2][BLK / 2];
+#pragma unroll
+    for(int i = 0; i < BLK / 2; ++i)
+    {
+      Raw row[BLK / 2];
+#pragma unroll
+      for(int j = 0; j < BLK / 2; ++j)
+      {
+        l1[i][j] = box<F, PRE>(F::template load<PRE>(tab, raw[2 * i][2 * j]),
+                               F::template load<PRE>(tab, raw[2 * i][2 * j + 1]),
+                               F::template load<PRE>(tab, raw[2 * i + 1][2 * j]),
+                               F::template load<PRE>(tab, raw[2 * i + 1][2 * j + 1]), fastOrder(COUNT, 1u));
+        row[j] = F::store(tab, l1[i][j]);
+      }
+      storeRow<F, BLK / 2>(texelPtr(img, inLevel + 1u, x0 >> 1, (y0 >> 1) + i, F::kBytes), row);
+    }
+    v = l1[0][0];
+    if constexpr(LB == 2)
+    {
+      v = box<F, false>(l1[0][0], l1[0][1], l1[1][0], l1[1][1], fastOrder(COUNT, 2u));
+      storeTexel<F>(texelPtr(img, inLevel + 2u, x0 >> 2, y0 >> 2, F::kBytes), F::store(tab, v));
+    }
+  }
+  if constexpr(G::UPPER)
+  {
+    __syncthreads();  // the previous duty wave has read level2[]
+    level2[threadIdx.x] = v;
+    __syncthreads();
+    if(wave == (iter & 3u))
+    {
+      // Duty lane k: level 3 from threads 4k..4k+3 (a Morton 2x2 of level 2).
+      constexpr int TEAM3 = 2 * (COUNT - 3);  // duty lanes per tile, log2 (<= 6)
+      const uint32_t t3   = g * G::TILES_PER_WG + (lane >> TEAM3);
+      const uint32_t k3   = lane & ((1u << TEAM3) - 1u);
+      const uint32_t qx   = compactEven(k3), qy = compactEven(k3 >> 1);
+      const V* c          = &level2[4u * lane];
+      V u                 = box<F, false>(c[0], c[1], c[2], c[3], fastOrder(COUNT, 3u));
+      if(t3 < numTiles)
+      {
+        const uint32_t ty = t3 / tilesX, tx = t3 - ty * tilesX;
+        storeTexel<F>(texelPtr(img, inLevel + 3u, (tx << (COUNT - 3)) + qx, (ty << (COUNT - 3)) + qy, F::kBytes),
+                      F::store(tab, u));
+#pragma unroll
+        for(int k = 0; k < 3; ++k)
+        {
+          const int r = 4 + k;
+          if(r > COUNT)
+            break;
+          if(k == 0)
+            u = crossLaneBox<F, 1>(u, fastOrder(COUNT, (uint32_t)r));
+          else if(k == 1)
+            u = crossLaneBox<F, 4>(u, fastOrder(COUNT, (uint32_t)r));
+          else
+            u = crossLaneBox<F, 16>(u, fastOrder(COUNT, (uint32_t)r));
+          if((k3 & ((4u << (2 * k)) - 1u)) == 0u)
+          {
+            const uint32_t sft = (uint32_t)(r - 3);
+            storeTexel<F>(texelPtr(img, inLevel + (uint32_t)r, (tx << (COUNT - r)) + (qx >> sft),
+                                   (ty << (COUNT - r)) + (qy >> sft), F::kBytes),
+                          F::store(tab, u));
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// General pipeline (nvpro_pyramid.glsl:536-884): 1 or 2 levels of any size with
+// the 2x2 / 2x3 / 3x2 / 3x3 energy-conserving kernel.
+// ---------------------------------------------------------------------------
+
+// Groups of a one-level general step (a thread per output) and of a two-level
+// one (an 8x8 tile of the second level each; *tilesX receives the tile columns).
+__host__ __device__ inline uint32_t generalGroupCount(uint32_t dstWidth, uint32_t dstHeight, uint32_t count,
+                                                      uint32_t* tilesX)
+{
+  if(count == 1u)
+  {
+    *tilesX = 0;
+    return (uint32_t)(((uint64_t)dstWidth * dstHeight + kGroupBS - 1) / kGroupBS);
+  }
+  *tilesX = (dstWidth + 7u) / 8u;
+  return *tilesX * ((dstHeight + 7u) / 8u);
+}
+
+// One level (glsl:828-842): outputs [256 g, 256 g + 256) of level inLevel+1, in
+// row-major order, a thread each.  No barrier.
+template <class F, class TT>
+__device__ __forceinline__ void generalOneGroup(const TT& tab, const ImageDesc& img, uint32_t inLevel, uint32_t g)
+{
+  const LevelDesc& src = img.lv[inLevel];
+  const LevelDesc& dst = img.lv[inLevel + 1u];
+  const int kx = kernelSize(src.width), ky = kernelSize(src.height);
+  const uint32_t i = g * kGroupBS + threadIdx.x;
+  if(i >= dst.width * dst.height)
+    return;
+  const uint32_t dy = i / dst.width, dx = i - dy * dst.width;
+  const uint8_t* p  = img.base + src.offset + (uint64_t)(2u * dy) * src.pitch + (uint64_t)(2u * dx) * F::kBytes;
+  auto at           = [&](int j, int r) {
+    return F::template load<false>(tab, loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)j * F::kBytes));
+  };
+  const typename F::Val v = generalSample<F>(at, kx, ky, dst.width, dst.height, dx, dy);
+  storeTexel<F>(texelPtr(img, inLevel + 1u, dx, dy, F::kBytes), F::store(tab, v));
+}
+
+// LDS values of a two-level general group: sharedLevel_ (nvpro_pyramid.glsl:555).
+constexpr uint32_t kGeneralMidEdge = 17;
+
+// Two levels: the 8x8 tile `tile` of level inLevel+2 and the first-level texels
+// it needs.  The first level's (up to) 17x17 tile is computed into `mid`
+// ([y][x], kGeneralMidEdge wide) and written out as well (fillIntermediateTile_,
+// glsl:732-781); then the 8x8 tile from LDS (fillLastTile_, glsl:791-820).
+// Neighbouring tiles recompute their shared first-level border texels and store
+// identical values.  Contains barriers.
+template <class F, class TT>
+__device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* mid, const ImageDesc& img,
+                                                uint32_t inLevel, uint32_t tilesX, uint32_t tile)
+{
+  constexpr uint32_t E = kGeneralMidEdge;
+  const LevelDesc& src = img.lv[inLevel];
+  const LevelDesc& l1  = img.lv[inLevel + 1u];
+  const LevelDesc& l2  = img.lv[inLevel + 2u];
+  const int kx1 = kernelSize(src.width), ky1 = kernelSize(src.height);
+  const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
+  const uint32_t nx = kx2 == 3 ? 17u : 16u, ny = ky2 == 3 ? 17u : 16u;
+  const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
+  for(uint32_t e = threadIdx.x; e < nx * ny; e += kGroupBS)
+  {
+    const uint32_t ly = e / nx, lx = e - ly * nx;
+    const uint32_t gx = 16u * tx + lx, gy = 16u * ty + ly;
+    if(gx < l1.width && gy < l1.height)
+    {
+      const uint8_t* p = img.base + src.offset + (uint64_t)(2u * gy) * src.pitch + (uint64_t)(2u * gx) * F::kBytes;
+      auto at          = [&](int j, int r) {
+        return F::template load<false>(tab, loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)j * F::kBytes));
+      };
+      const typename F::Val v = generalSample<F>(at, kx1, ky1, l1.width, l1.height, gx, gy);
+      storeTexel<F>(texelPtr(img, inLevel + 1u, gx, gy, F::kBytes), F::store(tab, v));
+      mid[ly * E + lx] = v;
+    }
+  }
+  __syncthreads();
+  if(threadIdx.x < 64u)
+  {
+    const uint32_t ox = threadIdx.x & 7u, oy = threadIdx.x >> 3;
+    const uint32_t gx = 8u * tx + ox, gy = 8u * ty + oy;
+    if(gx < l2.width && gy < l2.height)
+    {
+      auto at                 = [&](int j, int r) { return mid[(2u * oy + r) * E + 2u * ox + j]; };
+      const typename F::Val v = generalSample<F>(at, kx2, ky2, l2.width, l2.height, gx, gy);
+      storeTexel<F>(texelPtr(img, inLevel + 2u, gx, gy, F::kBytes), F::store(tab, v));
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Tail: every remaining step of a plan in one workgroup.
+//
+// Once a plan reaches small levels, one kernel per step only buys launch
+// latency.  A tail runs the remaining steps, fast and general alike: every level
+// is computed by the whole workgroup, stored to memory, and kept in LDS for the
+// next level; at a step boundary the value kept is the decode of the stored
+// texel, exactly what the next dispatch would load.  The arithmetic per level is
+// that of the per-step bodies (fast: box with the reference's order; general:
+// reduceStoreSample_), so results do not depend on which kernel ran a step.
+// `buf0`, `buf1`: LDS ping-pong of maxTexels values each, where maxTexels is at
+// least the first output level's texel count.  Contains barriers.
+// ---------------------------------------------------------------------------
+template <class F, class TT>
+__device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, typename F::Val* buf1,
+                                        const ImageDesc& img, const TailPlan& plan)
+{
+  using V = typename F::Val;
+  const V* cur = nullptr;  // the current source level in LDS, null: read it from memory
+  V* dst       = buf0;
+  for(uint32_t s = 0; s < plan.count; ++s)
+  {
+    const uint32_t L     = plan.steps[s].inputLevel;
+    const uint32_t count = plan.steps[s].levelCount;
+    const bool fast      = plan.steps[s].pipeline == NVPRO_PYRAMID_PIPELINE_FAST;
+    for(uint32_t r = 1; r <= count; ++r)
+    {
+      const LevelDesc& sl = img.lv[L + r - 1u];
+      const LevelDesc& dl = img.lv[L + r];
+      const int kx = kernelSize(sl.width), ky = kernelSize(sl.height);
+      const int order     = fastOrder(count, r);
+      const uint32_t n    = dl.width * dl.height;
+      for(uint32_t i = threadIdx.x; i < n; i += kGroupBS)
+      {
+        const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
+        auto at           = [&](int j, int k) -> V {
+          const uint32_t x = 2u * dx + (uint32_t)j, y = 2u * dy + (uint32_t)k;
+          if(!cur)
+          {
+            const uint8_t* p = img.base + sl.offset + (uint64_t)y * sl.pitch + (uint64_t)x * F::kBytes;
+            return F::template load<false>(tab, loadTexel<F>(p));
+          }
+          return cur[y * sl.width + x];
+        };
+        V v;
+        if(fast)
+          v = box<F, false>(at(0, 0), at(1, 0), at(0, 1), at(1, 1), order);
+        else
+          v = generalSample<F>(at, kx, ky, dl.width, dl.height, dx, dy);
+        const typename F::Raw q = F::store(tab, v);
+        storeTexel<F>(img.base + dl.offset + (uint64_t)dy * dl.pitch + (uint64_t)dx * F::kBytes, q);
+        if(r == count)
+          v = F::template load<false>(tab, q);  // the next step reads the stored texel
+        dst[i] = v;
+      }
+      __syncthreads();
+      cur = dst;
+      dst = dst == buf0 ? buf1 : buf0;
+    }
+  }
+}
+
+// Texel count of level `inputLevel + 1` (the first output level of a step).
+__host__ __device__ inline uint64_t firstOutputTexels(const ImageDesc& img, uint32_t inputLevel)
+{
+  return (uint64_t)img.lv[inputLevel + 1u].width * img.lv[inputLevel + 1u].height;
+}
+
+}  // namespace nvpro_pyramid

@@ -115,22 +115,6 @@ __device__ __forceinline__ uint32_t compactEven(uint32_t v)
   return v;
 }
 
-// Segment of a batched launch that holds group g (segments sorted by groupStart;
-// g is uniform, so this binary search runs on scalar loads).
-__device__ __forceinline__ uint32_t findSegment(const BatchSeg* segs, uint32_t n, uint32_t g)
-{
-  uint32_t lo = 0, hi = n;
-  while(hi - lo > 1u)
-  {
-    const uint32_t mid = (lo + hi) >> 1;
-    if(segs[mid].groupStart <= g)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 // kernelSizeFromInputSize_ (nvpro_pyramid.glsl:557-561).
 __host__ __device__ __forceinline__ int kernelSize(uint32_t dim) { return dim == 1u ? 1 : (int)(2u | (dim & 1u)); }
 

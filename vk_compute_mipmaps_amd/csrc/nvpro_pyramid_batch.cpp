@@ -1,0 +1,506 @@
+// C-ABI of batched chains (nvproPyramidBatch*): planning of the work list,
+// device upload, launches (direct or through a hipGraph) and the autotuner.
+//
+// Every image is planned exactly as nvproPyramidDispatch plans it (default
+// dispatchers, nvpro_pyramid_dispatch.hpp:109-304 of the reference).  Its
+// steps become segments; a tail segment replaces the small end of the plan the
+// way the per-image executor does.  Segments are cut into items of about
+// itemBytes input bytes.  Round r holds step r of every image; the work list is
+// round-major, largest segments first, so an item never depends on a later one.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "groups.hpp"
+
+using namespace nvpro_pyramid;
+
+namespace {
+
+constexpr uint32_t kDefaultItemBytes = 65536;
+constexpr uint32_t kSpinLimit        = 1u << 20;  // polls of >= 50 ns: far beyond any legitimate wait
+
+struct Plan
+{
+  std::vector<ImageDesc> imgs;
+  std::vector<QueueSeg> segs;
+  std::vector<QueueItem> items;
+  std::vector<TailPlan> tails;
+  std::vector<uint32_t> roundEnd;  // items [roundEnd[r-1], roundEnd[r]) form round r
+  uint32_t numSlots        = 0;
+  uint64_t algorithmicBytes = 0;
+};
+
+// Input bytes one group of a segment reads (for item sizing).
+uint64_t groupInputBytes(uint32_t kind, uint32_t bpt)
+{
+  if(kind == 1u)
+    return (uint64_t)kGroupBS * 4u * bpt;  // 2x2 per lane
+  if(kind <= 6u)
+    return (uint64_t)kGroupBS * 16u * bpt;  // 4x4 per lane
+  if(kind == kSegGeneral1)
+    return (uint64_t)kGroupBS * 4u * bpt;
+  if(kind == kSegGeneral2)
+    return 34ull * 34ull * bpt;
+  return 1u;
+}
+
+int checkConfig(const NvproPyramidBatchConfig& c, const char* who)
+{
+  if(c.mode > NVPRO_PYRAMID_BATCH_MODE_QUEUE)
+    return hostFail(hipErrorInvalidValue, "%s: unknown mode %u", who, c.mode);
+  if(c.tailTexels > kBatchTailTexels)
+    return hostFail(hipErrorInvalidValue, "%s: tailTexels %u above %u", who, c.tailTexels, kBatchTailTexels);
+  if(c.reserved[0] || c.reserved[1] || c.reserved[2])
+    return hostFail(hipErrorInvalidValue, "%s: reserved config fields must be 0", who);
+  return 0;
+}
+
+int buildPlan(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* images, const uint32_t* mipLevels,
+              uint32_t count, const NvproPyramidBatchConfig& cfg, Plan& plan, const char* who)
+{
+  if(!images || count == 0)
+    return hostFail(hipErrorInvalidValue, "%s: no images", who);
+  const uint32_t bpt       = pipelines ? nvproPyramidBytesPerTexel(pipelines->format) : 0u;
+  const uint32_t itemBytes = cfg.itemBytes ? cfg.itemBytes : kDefaultItemBytes;
+  plan                     = Plan{};
+  plan.imgs.resize(count);
+  // Per image: its steps as segments (before item cutting) in step order.
+  struct Pending
+  {
+    QueueSeg seg;
+    uint32_t groups, step;
+  };
+  std::vector<std::vector<Pending>> perImage(count);
+  uint32_t rounds = 0;
+  for(uint32_t i = 0; i < count; ++i)
+  {
+    NvproPyramidStep steps[kMaxPlanSteps];
+    uint32_t n = 0;
+    if(int e = prepareImage(pipelines, &images[i], mipLevels ? mipLevels[i] : 0u, nullptr, nullptr, &plan.imgs[i],
+                            steps, &n, who))
+      return e;
+    const ImageDesc& d = plan.imgs[i];
+    if(n == 0)
+      continue;
+    const uint32_t lastLevel = steps[n - 1].inputLevel + steps[n - 1].levelCount;
+    plan.algorithmicBytes += (uint64_t)d.lv[0].width * d.lv[0].height * bpt;
+    for(uint32_t l = 1; l <= lastLevel; ++l)
+      plan.algorithmicBytes += (uint64_t)d.lv[l].width * d.lv[l].height * bpt;
+    for(uint32_t s = 0; s < n; ++s)
+    {
+      const NvproPyramidStep& st = steps[s];
+      QueueSeg q{};
+      q.image = i;
+      q.depSlot = q.doneSlot = kNoSlot;
+      if(cfg.tailTexels && firstOutputTexels(d, st.inputLevel) <= cfg.tailTexels)
+      {
+        TailPlan tp{};
+        tp.image = i;
+        for(uint32_t k = s; k < n; ++k)
+          tp.steps[tp.count++] = TailStep{steps[k].pipeline, steps[k].inputLevel, steps[k].levelCount, 0u};
+        q.kind       = kSegTail;
+        q.inputLevel = (uint32_t)plan.tails.size();
+        plan.tails.push_back(tp);
+        perImage[i].push_back(Pending{q, 1u, s});
+        break;
+      }
+      q.inputLevel = st.inputLevel;
+      uint32_t groups;
+      if(st.pipeline == NVPRO_PYRAMID_PIPELINE_FAST)
+      {
+        const LevelDesc& in = d.lv[st.inputLevel];
+        q.kind              = st.levelCount;
+        q.tilesX            = in.width >> st.levelCount;
+        q.numTiles          = q.tilesX * (in.height >> st.levelCount);
+        groups              = fastGroupCount(in.width, in.height, st.levelCount);
+      }
+      else
+      {
+        const LevelDesc& out = d.lv[st.inputLevel + st.levelCount];
+        q.kind               = st.levelCount == 1u ? kSegGeneral1 : kSegGeneral2;
+        groups               = generalGroupCount(out.width, out.height, st.levelCount, &q.tilesX);
+        q.numTiles           = groups;
+      }
+      perImage[i].push_back(Pending{q, groups, s});
+    }
+    rounds = std::max(rounds, (uint32_t)perImage[i].size());
+  }
+  // Completion slots: one per segment that has a successor.  Segment ids are
+  // assigned round by round below; slots per (image, step position).
+  std::vector<std::vector<uint32_t>> slotOf(count), itemsOf(count);
+  for(uint32_t i = 0; i < count; ++i)
+  {
+    slotOf[i].assign(perImage[i].size(), kNoSlot);
+    itemsOf[i].assign(perImage[i].size(), 0u);
+    for(size_t k = 0; k + 1 < perImage[i].size(); ++k)
+      slotOf[i][k] = plan.numSlots++;
+  }
+  for(uint32_t r = 0; r < rounds; ++r)
+  {
+    std::vector<std::pair<uint32_t, uint32_t>> order;  // (groups, image)
+    for(uint32_t i = 0; i < count; ++i)
+      if(r < perImage[i].size())
+        order.push_back({perImage[i][r].groups, i});
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+                       return a.first > b.first;
+                     });
+    for(const auto& o : order)
+    {
+      const uint32_t i = o.second;
+      Pending& p       = perImage[i][r];
+      QueueSeg q       = p.seg;
+      if(r > 0)
+      {
+        q.depSlot  = slotOf[i][r - 1];
+        q.depCount = itemsOf[i][r - 1];
+      }
+      q.doneSlot             = slotOf[i][r];
+      const uint32_t segId   = (uint32_t)plan.segs.size();
+      const uint64_t perItem = std::max<uint64_t>(1u, itemBytes / groupInputBytes(q.kind, bpt));
+      uint32_t nItems        = 0;
+      for(uint64_t g = 0; g < p.groups; g += perItem, ++nItems)
+        plan.items.push_back(
+            QueueItem{segId, (uint32_t)g, (uint32_t)std::min<uint64_t>(perItem, p.groups - g), 0u});
+      itemsOf[i][r] = nItems;
+      plan.segs.push_back(q);
+    }
+    plan.roundEnd.push_back((uint32_t)plan.items.size());
+  }
+  return 0;
+}
+
+size_t align256(size_t v) { return (v + 255u) & ~size_t(255u); }
+
+}  // namespace
+
+struct NvproPyramidBatch
+{
+  NvproPyramidPipelines pipelines{};
+  NvproPyramidBatchConfig cfg{};
+  Plan plan;
+  void* dev          = nullptr;
+  size_t devBytes    = 0;
+  BatchArgs args{};
+  uint32_t blocks    = 0;  // workgroups of a queue launch / cap of a round launch
+  uint32_t launches  = 0;
+  uint32_t maxBlocks = 0;
+  hipStream_t priv   = nullptr;
+  hipGraph_t graph   = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static hipError_t enqueueBatch(NvproPyramidBatch* b, hipStream_t stream)
+{
+  const bool queue = b->cfg.mode == NVPRO_PYRAMID_BATCH_MODE_QUEUE;
+  if(b->plan.items.empty())
+    return hipSuccess;
+  if(queue)
+  {
+    BatchArgs a  = b->args;
+    a.itemBegin = 0;
+    a.itemEnd   = (uint32_t)b->plan.items.size();
+    return launchBatch(stream, b->pipelines.format, b->pipelines.arithmetic, true, a, b->blocks);
+  }
+  uint32_t begin = 0;
+  for(uint32_t end : b->plan.roundEnd)
+  {
+    if(end > begin)
+    {
+      BatchArgs a  = b->args;
+      a.itemBegin = begin;
+      a.itemEnd   = end;
+      const uint32_t n = std::min(end - begin, b->blocks);
+      if(hipError_t e = launchBatch(stream, b->pipelines.format, b->pipelines.arithmetic, false, a, n))
+        return e;
+    }
+    begin = end;
+  }
+  return hipSuccess;
+}
+
+static void releaseBatch(NvproPyramidBatch* b)
+{
+  if(b->exec)
+    (void)hipGraphExecDestroy(b->exec);
+  if(b->graph)
+    (void)hipGraphDestroy(b->graph);
+  if(b->priv)
+    (void)hipStreamSynchronize(b->priv);
+  if(b->dev)
+    (void)hipFree(b->dev);
+  if(b->priv)
+    (void)hipStreamDestroy(b->priv);
+  delete b;
+}
+
+extern "C" {
+
+int nvproPyramidBatchDefaultConfig(NvproPyramidBatchConfig* config)
+{
+  if(!config)
+    return hostFail(hipErrorInvalidValue, "nvproPyramidBatchDefaultConfig: null config");
+  memset(config, 0, sizeof *config);
+  config->mode            = NVPRO_PYRAMID_BATCH_MODE_QUEUE;
+  config->workgroupsPerCU = 0;
+  config->itemBytes       = kDefaultItemBytes;
+  config->tailTexels      = kBatchTailTexels;
+  config->useGraph        = 1;
+  return 0;
+}
+
+int nvproPyramidBatchCreate(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* images,
+                            const uint32_t* mipLevels, uint32_t imageCount, const NvproPyramidBatchConfig* config,
+                            NvproPyramidBatch** batch)
+{
+  const char* who = "nvproPyramidBatchCreate";
+  if(!batch)
+    return hostFail(hipErrorInvalidValue, "%s: null output", who);
+  *batch = nullptr;
+  NvproPyramidBatchConfig cfg;
+  nvproPyramidBatchDefaultConfig(&cfg);
+  if(config)
+    cfg = *config;
+  if(int e = checkConfig(cfg, who))
+    return e;
+  NvproPyramidBatch* b = new NvproPyramidBatch;
+  b->cfg               = cfg;
+  b->pipelines         = pipelines ? *pipelines : NvproPyramidPipelines{};
+  if(int e = buildPlan(pipelines, images, mipLevels, imageCount, cfg, b->plan, who))
+  {
+    delete b;
+    return e;
+  }
+  const Plan& p = b->plan;
+  // Device layout: images | segments | items | tails | counters.
+  const size_t oImg = 0;
+  const size_t oSeg = align256(oImg + p.imgs.size() * sizeof(ImageDesc));
+  const size_t oItm = align256(oSeg + p.segs.size() * sizeof(QueueSeg));
+  const size_t oTl  = align256(oItm + p.items.size() * sizeof(QueueItem));
+  const size_t oCnt = align256(oTl + p.tails.size() * sizeof(TailPlan));
+  b->devBytes       = align256(oCnt + (kCounterSlots + p.numSlots) * sizeof(uint32_t));
+  std::vector<uint8_t> host(b->devBytes, 0);
+  memcpy(host.data() + oImg, p.imgs.data(), p.imgs.size() * sizeof(ImageDesc));
+  if(!p.segs.empty())
+    memcpy(host.data() + oSeg, p.segs.data(), p.segs.size() * sizeof(QueueSeg));
+  if(!p.items.empty())
+    memcpy(host.data() + oItm, p.items.data(), p.items.size() * sizeof(QueueItem));
+  if(!p.tails.empty())
+    memcpy(host.data() + oTl, p.tails.data(), p.tails.size() * sizeof(TailPlan));
+  hipError_t e = hipStreamCreateWithFlags(&b->priv, hipStreamNonBlocking);
+  if(e == hipSuccess)
+    e = hipMalloc(&b->dev, b->devBytes);
+  if(e == hipSuccess)
+    e = hipMemcpyAsync(b->dev, host.data(), b->devBytes, hipMemcpyHostToDevice, b->priv);
+  if(e == hipSuccess)
+    e = hipStreamSynchronize(b->priv);
+  if(e != hipSuccess)
+  {
+    releaseBatch(b);
+    return hostFail(e, "%s: device work list: %s", who, hipGetErrorString(e));
+  }
+  uint8_t* base    = (uint8_t*)b->dev;
+  b->args.imgs     = (const ImageDesc*)(base + oImg);
+  b->args.segs     = (const QueueSeg*)(base + oSeg);
+  b->args.items    = (const QueueItem*)(base + oItm);
+  b->args.tails    = (const TailPlan*)(base + oTl);
+  b->args.counters = (uint32_t*)(base + oCnt);
+  b->args.numSlots = p.numSlots;
+  b->args.spinLimit = kSpinLimit;
+  const bool queue     = cfg.mode == NVPRO_PYRAMID_BATCH_MODE_QUEUE;
+  const uint32_t fit   = batchMaxWorkgroupsPerCU(b->pipelines.format, b->pipelines.arithmetic, queue);
+  const uint32_t perCU = cfg.workgroupsPerCU ? std::min(cfg.workgroupsPerCU, fit) : fit;
+  const uint64_t cap   = (uint64_t)deviceComputeUnits() * perCU;
+  b->blocks            = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(cap, p.items.size()));
+  b->launches          = 0;
+  uint32_t begin       = 0;
+  for(uint32_t end : p.roundEnd)
+  {
+    if(end > begin)
+      b->launches += 1u;
+    begin = end;
+  }
+  if(queue && !p.items.empty())
+    b->launches = 1u;
+  if(cfg.useGraph && !p.items.empty())
+  {
+    e = hipStreamBeginCapture(b->priv, hipStreamCaptureModeThreadLocal);
+    if(e == hipSuccess)
+    {
+      hipError_t le = enqueueBatch(b, b->priv);
+      e             = hipStreamEndCapture(b->priv, &b->graph);
+      if(le != hipSuccess)
+        e = le;
+    }
+    if(e == hipSuccess)
+      e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if(e != hipSuccess)
+    {
+      releaseBatch(b);
+      return hostFail(e, "%s: graph: %s", who, hipGetErrorString(e));
+    }
+  }
+  *batch = b;
+  hostClearError();
+  return 0;
+}
+
+int nvproPyramidBatchLaunch(NvproPyramidBatch* batch, hipStream_t stream)
+{
+  if(!batch)
+    return hostFail(hipErrorInvalidValue, "nvproPyramidBatchLaunch: null batch");
+  hipError_t e = batch->exec ? hipGraphLaunch(batch->exec, stream) : enqueueBatch(batch, stream);
+  if(e != hipSuccess)
+    return hostFail(e, "nvproPyramidBatchLaunch: %s", hipGetErrorString(e));
+  return 0;
+}
+
+int nvproPyramidBatchGetInfo(const NvproPyramidBatch* batch, NvproPyramidBatchInfo* info)
+{
+  if(!batch || !info)
+    return hostFail(hipErrorInvalidValue, "nvproPyramidBatchGetInfo: null argument");
+  memset(info, 0, sizeof *info);
+  info->imageCount       = (uint32_t)batch->plan.imgs.size();
+  info->segmentCount     = (uint32_t)batch->plan.segs.size();
+  info->itemCount        = (uint32_t)batch->plan.items.size();
+  info->roundCount       = (uint32_t)batch->plan.roundEnd.size();
+  info->launchCount      = batch->launches;
+  info->workgroups       = batch->plan.items.empty() ? 0u : batch->blocks;
+  info->algorithmicBytes = batch->plan.algorithmicBytes;
+  info->deviceBytes      = batch->devBytes;
+  info->config           = batch->cfg;
+  return 0;
+}
+
+int nvproPyramidBatchReadErrors(NvproPyramidBatch* batch, uint32_t* flags)
+{
+  if(!batch || !flags)
+    return hostFail(hipErrorInvalidValue, "nvproPyramidBatchReadErrors: null argument");
+  hipError_t e = hipDeviceSynchronize();
+  uint32_t v   = 0;
+  uint32_t* p  = batch->args.counters + kCounterError;
+  if(e == hipSuccess)
+    e = hipMemcpyAsync(&v, p, sizeof v, hipMemcpyDeviceToHost, batch->priv);
+  const uint32_t zero = 0;
+  if(e == hipSuccess)
+    e = hipMemcpyAsync(p, &zero, sizeof zero, hipMemcpyHostToDevice, batch->priv);
+  if(e == hipSuccess)
+    e = hipStreamSynchronize(batch->priv);
+  if(e != hipSuccess)
+    return hostFail(e, "nvproPyramidBatchReadErrors: %s", hipGetErrorString(e));
+  *flags = v;
+  return 0;
+}
+
+int nvproPyramidBatchDestroy(NvproPyramidBatch* batch)
+{
+  if(batch)
+    releaseBatch(batch);
+  return 0;
+}
+
+int nvproPyramidBatchAutotune(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* images,
+                              const uint32_t* mipLevels, uint32_t imageCount, hipStream_t stream,
+                              uint32_t repetitions, NvproPyramidBatchConfig* best, float* bestMicroseconds)
+{
+  const char* who = "nvproPyramidBatchAutotune";
+  if(!best)
+    return hostFail(hipErrorInvalidValue, "%s: null output", who);
+  if(!pipelines || !images || imageCount == 0)
+    return hostFail(hipErrorInvalidValue, "%s: no images", who);
+  // Cache key: device, format, arithmetic, image count and total base texels.
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  uint64_t texels = 0;
+  for(uint32_t i = 0; i < imageCount; ++i)
+    texels += (uint64_t)images[i].width * images[i].height;
+  using Key = std::tuple<int, uint32_t, uint32_t, uint32_t, uint64_t>;
+  static std::mutex mtx;
+  static std::map<Key, std::pair<NvproPyramidBatchConfig, float>> cache;
+  const Key key{dev, pipelines->format, pipelines->arithmetic, imageCount, texels};
+  {
+    std::lock_guard<std::mutex> lock(mtx);
+    auto it = cache.find(key);
+    if(it != cache.end())
+    {
+      *best = it->second.first;
+      if(bestMicroseconds)
+        *bestMicroseconds = it->second.second;
+      return 0;
+    }
+  }
+  const uint32_t reps = repetitions ? repetitions : 5u;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if(he == hipSuccess)
+    he = hipEventCreate(&e1);
+  if(he != hipSuccess)
+    return hostFail(he, "%s: events: %s", who, hipGetErrorString(he));
+  float bestUs = INFINITY;
+  NvproPyramidBatchConfig winner;
+  nvproPyramidBatchDefaultConfig(&winner);
+  int status = 0;
+  for(uint32_t mode : {NVPRO_PYRAMID_BATCH_MODE_QUEUE, NVPRO_PYRAMID_BATCH_MODE_ROUNDS})
+    for(uint32_t perCU : {0u, 4u})
+      for(uint32_t itemBytes : {16384u, 65536u, 262144u})
+        for(uint32_t tail : {0u, kBatchTailTexels})
+        {
+          NvproPyramidBatchConfig c;
+          nvproPyramidBatchDefaultConfig(&c);
+          c.mode            = mode;
+          c.workgroupsPerCU = perCU;
+          c.itemBytes       = itemBytes;
+          c.tailTexels      = tail;
+          NvproPyramidBatch* b = nullptr;
+          if((status = nvproPyramidBatchCreate(pipelines, images, mipLevels, imageCount, &c, &b)) != 0)
+            goto done;
+          status = nvproPyramidBatchLaunch(b, stream);  // warm-up
+          if(status == 0 && (he = hipEventRecord(e0, stream)) != hipSuccess)
+            status = hostFail(he, "%s: %s", who, hipGetErrorString(he));
+          for(uint32_t r = 0; status == 0 && r < reps; ++r)
+            status = nvproPyramidBatchLaunch(b, stream);
+          if(status == 0 && (he = hipEventRecord(e1, stream)) != hipSuccess)
+            status = hostFail(he, "%s: %s", who, hipGetErrorString(he));
+          if(status == 0 && (he = hipEventSynchronize(e1)) != hipSuccess)
+            status = hostFail(he, "%s: %s", who, hipGetErrorString(he));
+          float ms = 0.0f;
+          if(status == 0 && (he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess)
+            status = hostFail(he, "%s: %s", who, hipGetErrorString(he));
+          uint32_t flags = 0;
+          if(status == 0)
+            status = nvproPyramidBatchReadErrors(b, &flags);
+          nvproPyramidBatchDestroy(b);
+          if(status != 0)
+            goto done;
+          const float us = ms * 1000.0f / (float)reps;
+          if(flags == 0 && us < bestUs)
+          {
+            bestUs = us;
+            winner = c;
+          }
+        }
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if(status != 0)
+    return status;
+  if(!(bestUs < INFINITY))
+    return hostFail(hipErrorUnknown, "%s: every candidate reported errors", who);
+  {
+    std::lock_guard<std::mutex> lock(mtx);
+    cache[key] = {winner, bestUs};
+  }
+  *best = winner;
+  if(bestMicroseconds)
+    *bestMicroseconds = bestUs;
+  hostClearError();
+  return 0;
+}
+
+}  // extern "C"

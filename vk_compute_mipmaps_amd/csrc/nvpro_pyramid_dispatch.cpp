@@ -20,16 +20,33 @@ constexpr uint32_t kMaxSteps = 64;  // far above the <= 15 a 16-level chain can 
 
 thread_local char gLastError[256] = "";
 
+int vfail(int code, const char* fmt, va_list ap)
+{
+  vsnprintf(gLastError, sizeof gLastError, fmt, ap);
+  return code;
+}
+
 int fail(int code, const char* fmt, ...)
 {
   va_list ap;
   va_start(ap, fmt);
-  vsnprintf(gLastError, sizeof gLastError, fmt, ap);
+  vfail(code, fmt, ap);
   va_end(ap);
   return code;
 }
 
 }  // namespace
+
+int nvpro_pyramid::hostFail(int code, const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vfail(code, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+void nvpro_pyramid::hostClearError() { gLastError[0] = 0; }
 
 struct NvproPyramidCmd
 {
@@ -355,27 +372,45 @@ static int bindImage(const NvproPyramidImage* image, uint32_t format, uint32_t a
   return 0;
 }
 
+int nvpro_pyramid::prepareImage(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* image,
+                                uint32_t mipLevels, nvpro_pyramid_dispatcher_t generalDispatcher,
+                                nvpro_pyramid_dispatcher_t fastDispatcher, ImageDesc* desc, NvproPyramidStep* steps,
+                                uint32_t* stepCount, const char* who)
+{
+  *stepCount = 0;
+  if(int e = checkPipelines(pipelines, who))
+    return e;
+  if(int e = bindImage(image, pipelines->format, pipelines->arithmetic, desc, who))
+    return e;
+  NvproPyramidCmd cmd;
+  if(int e = initCmd(&cmd, pipelines, image->width, image->height, mipLevels, desc->levelCount, who))
+    return e;
+  if(cmd.mipLevels <= 1u)
+    return 0;
+  if(int e = runDispatchLoop(&cmd, generalDispatcher, fastDispatcher))
+    return e;
+  if(cmd.stepCount > kMaxPlanSteps)
+    return fail(hipErrorInvalidValue, "%s: plan of %u steps", who, cmd.stepCount);
+  for(uint32_t i = 0; i < cmd.stepCount; ++i)
+    steps[i] = cmd.steps[i];
+  *stepCount = cmd.stepCount;
+  return 0;
+}
+
 extern "C" int nvproPyramidDispatchCustom(hipStream_t stream, const NvproPyramidPipelines* pipelines,
                                           const NvproPyramidImage* image, uint32_t mipLevels,
                                           nvpro_pyramid_dispatcher_t generalDispatcher,
                                           nvpro_pyramid_dispatcher_t fastDispatcher)
 {
   const char* who = "nvproPyramidDispatch";
-  if(int e = checkPipelines(pipelines, who))
-    return e;
   ImageDesc desc;
-  if(int e = bindImage(image, pipelines->format, pipelines->arithmetic, &desc, who))
+  NvproPyramidStep steps[kMaxPlanSteps];
+  uint32_t count = 0;
+  if(int e = prepareImage(pipelines, image, mipLevels, generalDispatcher, fastDispatcher, &desc, steps, &count, who))
     return e;
-  NvproPyramidCmd cmd;
-  if(int e = initCmd(&cmd, pipelines, image->width, image->height, mipLevels, desc.levelCount, who))
-    return e;
-  if(cmd.mipLevels <= 1u)
-    return 0;
-  if(int e = runDispatchLoop(&cmd, generalDispatcher, fastDispatcher))
-    return e;
-  if(cmd.stepCount != 0)
+  if(count != 0)
   {
-    hipError_t e = executePlan(stream, desc, cmd.steps, cmd.stepCount);
+    hipError_t e = executePlan(stream, desc, steps, count);
     if(e != hipSuccess)
       return fail(e, "%s: kernel launch failed: %s", who, hipGetErrorString(e));
   }

@@ -40,17 +40,38 @@ struct TailPlan
   TailStep steps[NVPRO_PYRAMID_MAX_LEVELS];
 };
 
-// One image's share of a batched launch: tile groups
-// [groupStart, next segment's groupStart) belong to step (inputLevel) of image.
-struct BatchSeg
-{
-  uint32_t image, inputLevel, tilesX, numTiles, groupStart, pad[3];
-};
-
 // A step whose first output level has at most kTailMaxTexels texels, and every
-// step after it, runs in the one-workgroup tail kernel of kTailBS threads.
+// step after it, runs in the one-workgroup tail kernel.
 constexpr uint32_t kTailMaxTexels = 1024;
-constexpr int kTailBS             = 256;
+
+// Batched work (batch_pipeline.hip): the steps of many images as segments, cut
+// into items of consecutive groups, ordered so that every item comes after the
+// items it depends on.
+enum SegKind : uint32_t
+{
+  kSegFast1     = 1,  // kinds 1..6: fast step filling that many levels
+  kSegGeneral1  = 7,
+  kSegGeneral2  = 8,
+  kSegTail      = 9,  // inputLevel holds the index of the TailPlan
+};
+constexpr uint32_t kNoSlot = 0xffffffffu;
+struct QueueSeg
+{
+  uint32_t image, kind, inputLevel, tilesX, numTiles;
+  uint32_t depSlot, depCount;  // wait until counter depSlot reaches depCount (kNoSlot: no wait)
+  uint32_t doneSlot;           // counter bumped once per finished item (kNoSlot: none)
+};
+struct QueueItem
+{
+  uint32_t seg, firstGroup, groupCount, pad;
+};
+// Counter block of a batch: head of the queue, workgroups exited, sticky error
+// flags, then one completion counter per slot.
+constexpr uint32_t kCounterHead = 0, kCounterExited = 1, kCounterError = 2, kCounterSlots = 4;
+constexpr uint32_t kBatchErrorSpin = 1u;  // a dependency wait gave up
+// Largest first output level a tail may have inside the batched kernel (its LDS
+// ping-pong overlays the other bodies' scratch).
+constexpr uint32_t kBatchTailTexels = 128;
 
 // Per-pipeline launchers.  The caller has validated every step.
 hipError_t launchFast(hipStream_t stream, const ImageDesc& img, uint32_t inputLevel, uint32_t levelCount,
@@ -58,18 +79,23 @@ hipError_t launchFast(hipStream_t stream, const ImageDesc& img, uint32_t inputLe
 hipError_t launchGeneral(hipStream_t stream, const ImageDesc& img, uint32_t inputLevel, uint32_t levelCount);
 hipError_t launchTail(hipStream_t stream, const ImageDesc& img, const TailPlan& plan);
 
-// Batched launchers: one launch covers the segments of many images (device
-// arrays).  Fast: all segments fill `count` levels.  General: `count` 1 or 2.
-hipError_t launchFastBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic, uint32_t count,
-                           const ImageDesc* imgs, const BatchSeg* segs, uint32_t numSegs, uint32_t totalGroups,
-                           uint32_t wgPerCU);
-hipError_t launchGeneralBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic, uint32_t count,
-                              const ImageDesc* imgs, const BatchSeg* segs, uint32_t numSegs, uint32_t totalGroups);
-hipError_t launchTailBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic, const ImageDesc* imgs,
-                           const TailPlan* plans, uint32_t numPlans);
-// Tile groups one launch of the fast / general kernel uses for a step.
-uint32_t fastGroupsPerStep(uint32_t width, uint32_t height, uint32_t count);
-uint32_t generalGroupsPerStep(uint32_t dstWidth, uint32_t dstHeight, uint32_t count, uint32_t* tilesX);
+// Batched kernel over items [itemBegin, itemEnd) of a work list in device
+// memory.  queue: one persistent launch that dequeues items and waits on their
+// dependencies (spinLimit bounds each wait); otherwise a launch of independent
+// items (one round).  `blocks` workgroups.
+struct BatchArgs
+{
+  const ImageDesc* imgs;
+  const QueueSeg* segs;
+  const QueueItem* items;
+  const TailPlan* tails;
+  uint32_t* counters;
+  uint32_t itemBegin, itemEnd, numSlots, spinLimit;
+};
+hipError_t launchBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic, bool queue, const BatchArgs& args,
+                       uint32_t blocks);
+// Resident workgroups per compute unit of the batched kernel.
+uint32_t batchMaxWorkgroupsPerCU(uint32_t format, uint32_t arithmetic, bool queue);
 
 // Launch a validated plan: per-step kernels for the large steps, then one tail
 // launch for the small remainder.
@@ -77,5 +103,17 @@ hipError_t executePlan(hipStream_t stream, const ImageDesc& img, const NvproPyra
 
 // Number of compute units of the current device (cached per device).
 uint32_t deviceComputeUnits();
+
+// Host helpers of the dispatcher (nvpro_pyramid_dispatch.cpp).
+// A plan has at most one step per level.
+constexpr uint32_t kMaxPlanSteps = NVPRO_PYRAMID_MAX_LEVELS;
+// Records a message for nvproPyramidGetLastError and returns code.
+int hostFail(int code, const char* fmt, ...);
+void hostClearError();
+// Validates pipelines and image, binds the image and runs the dispatcher loop:
+// *desc and steps[0, *stepCount) (capacity kMaxPlanSteps).  0 or an error.
+int prepareImage(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* image, uint32_t mipLevels,
+                 nvpro_pyramid_dispatcher_t generalDispatcher, nvpro_pyramid_dispatcher_t fastDispatcher,
+                 ImageDesc* desc, NvproPyramidStep* steps, uint32_t* stepCount, const char* who);
 
 }  // namespace nvpro_pyramid

@@ -44,7 +44,20 @@ EXPORTED_SYMBOLS = (
     "nvproPyramidPackedLayout",
     "nvproPyramidBytesPerTexel",
     "nvproPyramidGetLastError",
+    "nvproPyramidSynthesize",
+    "nvproPyramidBatchDefaultConfig",
+    "nvproPyramidBatchCreate",
+    "nvproPyramidBatchLaunch",
+    "nvproPyramidBatchGetInfo",
+    "nvproPyramidBatchReadErrors",
+    "nvproPyramidBatchDestroy",
+    "nvproPyramidBatchAutotune",
 )
+
+BATCH_MODE_ROUNDS = 0
+BATCH_MODE_QUEUE = 1
+BATCH_ERROR_SPIN = 1
+BATCH_MAX_TAIL_TEXELS = 128
 
 
 class NvproPyramidError(RuntimeError):
@@ -81,6 +94,25 @@ class NvproPyramidStep(ctypes.Structure):
     def __repr__(self):
         kind = "F" if self.pipeline == PIPELINE_FAST else "G"
         return f"{kind}({self.inputLevel},{self.levelCount},{self.groupCountX})"
+
+
+class NvproPyramidBatchConfig(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_uint32), ("workgroupsPerCU", ctypes.c_uint32),
+                ("itemBytes", ctypes.c_uint32), ("tailTexels", ctypes.c_uint32),
+                ("useGraph", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+    def as_dict(self):
+        return {"mode": "queue" if self.mode == BATCH_MODE_QUEUE else "rounds",
+                "workgroupsPerCU": self.workgroupsPerCU, "itemBytes": self.itemBytes,
+                "tailTexels": self.tailTexels, "useGraph": bool(self.useGraph)}
+
+
+class NvproPyramidBatchInfo(ctypes.Structure):
+    _fields_ = [("imageCount", ctypes.c_uint32), ("segmentCount", ctypes.c_uint32),
+                ("itemCount", ctypes.c_uint32), ("roundCount", ctypes.c_uint32),
+                ("launchCount", ctypes.c_uint32), ("workgroups", ctypes.c_uint32),
+                ("algorithmicBytes", ctypes.c_uint64), ("deviceBytes", ctypes.c_uint64),
+                ("config", NvproPyramidBatchConfig)]
 
 
 DISPATCHER = ctypes.CFUNCTYPE(ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(NvproPyramidState))
@@ -121,6 +153,22 @@ def lib():
             getattr(L, name).argtypes = [vp, ctypes.POINTER(NvproPyramidState)]
         L.nvproPyramidFastDispatcherEx.restype = u32
         L.nvproPyramidFastDispatcherEx.argtypes = [vp, ctypes.POINTER(NvproPyramidState), u32, u32]
+        L.nvproPyramidSynthesize.restype = ctypes.c_int
+        L.nvproPyramidSynthesize.argtypes = [vp, u32, ctypes.POINTER(NvproPyramidImage), ctypes.c_uint64]
+        cfgp = ctypes.POINTER(NvproPyramidBatchConfig)
+        for name, args in (
+                ("nvproPyramidBatchDefaultConfig", [cfgp]),
+                ("nvproPyramidBatchCreate", [ctypes.POINTER(NvproPyramidPipelines), ctypes.POINTER(NvproPyramidImage),
+                                             ctypes.POINTER(u32), u32, cfgp, ctypes.POINTER(vp)]),
+                ("nvproPyramidBatchLaunch", [vp, vp]),
+                ("nvproPyramidBatchGetInfo", [vp, ctypes.POINTER(NvproPyramidBatchInfo)]),
+                ("nvproPyramidBatchReadErrors", [vp, ctypes.POINTER(u32)]),
+                ("nvproPyramidBatchDestroy", [vp]),
+                ("nvproPyramidBatchAutotune", [ctypes.POINTER(NvproPyramidPipelines),
+                                               ctypes.POINTER(NvproPyramidImage), ctypes.POINTER(u32), u32, vp, u32,
+                                               cfgp, ctypes.POINTER(ctypes.c_float)])):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = args
         _lib = L
     return _lib
 
@@ -228,6 +276,11 @@ def nvpro_cmd_pyramid_dispatch_custom(stream: int, pipes: NvproPyramidPipelines,
                                             mip_levels, _fn_ptr(general_dispatcher), _fn_ptr(fast_dispatcher)))
 
 
+def synthesize(stream: int, fmt: int, image: NvproPyramidImage, seed: int):
+    """nvproPyramidSynthesize: seeded synthetic level 0 of `image` (synth.py's spec)."""
+    _check(lib().nvproPyramidSynthesize(ctypes.c_void_p(stream), fmt, ctypes.byref(image), seed))
+
+
 POISON_SRGBA8 = 0xFFFF00FF  # magenta, opaque (bytes FF 00 FF FF), as the reference's test clears
 
 
@@ -254,6 +307,10 @@ class DeviceMipChain:
         a = np.ascontiguousarray(chain)
         assert a.nbytes == self.nbytes
         self.rt.memcpy_htod(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+
+    def synthesize(self, seed: int, stream: int = 0):
+        """Write the seeded synthetic level 0 (synth.py's images) on the device."""
+        synthesize(stream, self.fmt, self.image, seed)
 
     def poison(self, word: int = POISON_SRGBA8, stream: int = 0):
         """Fill levels >= 1 with a poison pattern (unwritten texels stay visible)."""
@@ -286,3 +343,75 @@ class DeviceMipChain:
             self.free()
         except Exception:
             pass
+
+
+def batch_config(mode: int = BATCH_MODE_QUEUE, workgroups_per_cu: int = 0, item_bytes: int = 65536,
+                 tail_texels: int = BATCH_MAX_TAIL_TEXELS, use_graph: bool = True) -> NvproPyramidBatchConfig:
+    c = NvproPyramidBatchConfig()
+    _check(lib().nvproPyramidBatchDefaultConfig(ctypes.byref(c)))
+    c.mode, c.workgroupsPerCU, c.itemBytes = mode, workgroups_per_cu, item_bytes
+    c.tailTexels, c.useGraph = tail_texels, 1 if use_graph else 0
+    return c
+
+
+def _image_array(images):
+    arr = (NvproPyramidImage * len(images))()
+    for i, im in enumerate(images):
+        arr[i] = im
+    return arr
+
+
+def _mip_array(mip_levels, n):
+    if mip_levels is None:
+        return None
+    return (ctypes.c_uint32 * n)(*mip_levels)
+
+
+class PyramidBatch:
+    """nvproPyramidBatch*: many images' chains in one persistent launch (or one per round)."""
+
+    def __init__(self, pipes: NvproPyramidPipelines, images, mip_levels=None,
+                 config: NvproPyramidBatchConfig | None = None):
+        self._images = _image_array(images)  # keeps offset/pitch arrays alive with the structs
+        self._keep = [getattr(im, "_keep", None) for im in images]
+        mips = _mip_array(mip_levels, len(images))
+        h = ctypes.c_void_p()
+        _check(lib().nvproPyramidBatchCreate(ctypes.byref(pipes), self._images, mips, len(images),
+                                             ctypes.byref(config) if config is not None else None,
+                                             ctypes.byref(h)))
+        self.handle = h.value
+
+    def launch(self, stream: int = 0):
+        _check(lib().nvproPyramidBatchLaunch(ctypes.c_void_p(self.handle), ctypes.c_void_p(stream)))
+
+    def info(self) -> NvproPyramidBatchInfo:
+        out = NvproPyramidBatchInfo()
+        _check(lib().nvproPyramidBatchGetInfo(ctypes.c_void_p(self.handle), ctypes.byref(out)))
+        return out
+
+    def read_errors(self) -> int:
+        flags = ctypes.c_uint32()
+        _check(lib().nvproPyramidBatchReadErrors(ctypes.c_void_p(self.handle), ctypes.byref(flags)))
+        return flags.value
+
+    def destroy(self):
+        if self.handle:
+            lib().nvproPyramidBatchDestroy(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def batch_autotune(pipes: NvproPyramidPipelines, images, mip_levels=None, stream: int = 0, repetitions: int = 5):
+    """The project autotuner: (best NvproPyramidBatchConfig, its microseconds per launch)."""
+    arr = _image_array(images)
+    best = NvproPyramidBatchConfig()
+    us = ctypes.c_float()
+    _check(lib().nvproPyramidBatchAutotune(ctypes.byref(pipes), arr, _mip_array(mip_levels, len(images)),
+                                           len(images), ctypes.c_void_p(stream), repetitions, ctypes.byref(best),
+                                           ctypes.byref(us)))
+    return best, us.value
