@@ -10,7 +10,12 @@ dispatches, after --warmup untimed ones.
 Printed (one JSON line): the metric, per-chain time, chains/s, the roofline of
 the dominant kernel (the 6-level fast-pipeline launch over level 0), and the
 reference's CPU generator (mipmap_storage.hpp:246-414, restated in C) timed on
-the host as a baseline.
+the host as a baseline.  Beside it:
+  batched   512 mixed-size sRGBA8 textures (seed 8): one batched launch against a
+            loop of plain per-texture dispatches, timed by the C++ host
+            (lib/nvpro_pyramid_demo batch), chains/s and p50/p99 per batch;
+  configs   single chains of the large configurations (16384^2 sRGBA8, 8192^2
+            RGBA16F with FP32 and packed-FP16 arithmetic): us and GB/s.
 """
 from __future__ import annotations
 
@@ -40,6 +45,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--only-first-step", action="store_true",
                     help="dispatch only the 6-level fast step over level 0 (for profiling)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     return ap.parse_args()
 
 
@@ -53,15 +60,16 @@ def main():
 
     import numpy as np
 
-    from vk_compute_mipmaps_amd import hip_rt, pyramid as P, synth
+    from vk_compute_mipmaps_amd import hip_rt, pyramid as P
 
     hip_rt.check(hip_rt.rt().hipSetDevice(0), "hipSetDevice")
     stream = hip_rt.stream_create()
-    level0 = synth.srgba8(W, H, SEED)
     chain_bytes = P.chain_bytes(W, H)
     n_buf = max(2, int(np.ceil(args.rotate_bytes / chain_bytes)))
     chains = [P.DeviceMipChain(W, H) for _ in range(n_buf)]
-    chains[0].upload_level0(level0)
+    chains[0].synthesize(SEED, stream)  # synth.srgba8(W, H, SEED), generated on the device
+    hip_rt.stream_sync(stream)
+    level0 = chains[0].download()[: W * H * 4]
     for c in chains[1:]:
         hip_rt.memcpy_dtod_async(c.ptr, chains[0].ptr, W * H * 4, stream)
     for c in chains:
@@ -133,7 +141,9 @@ def main():
         hip_rt.event_destroy(e)
     for c in chains:
         c.free()
+    configs = None if args.no_configs else large_configs(stream)
     hip_rt.stream_destroy(stream)
+    batched = None if args.no_batched else batched_leg()
 
     traffic = None
     tpath = os.path.join(REPO, "profiles", "fast_pipeline_4096_traffic.json")
@@ -174,7 +184,7 @@ def main():
         "chain_roofline_frac": round(chain_total_bytes / (us_per_chain * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {
             "bound": "hbm",
-            "kernel": "fastPipelineKernel<Srgba8,3> (F(0,6) over level 0)",
+            "kernel": "fastPipelineKernel<Srgba8,6> (F(0,6) over level 0)",
             "achieved": round(k_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -185,8 +195,54 @@ def main():
         },
         "copy_GBps": round(copy_gbs, 1),
         "cpu_baseline": cpu,
+        "batched": batched,
+        "configs": configs,
     }
     print(json.dumps(out))
+
+
+def batched_leg():
+    """512 mixed textures: batched launch vs a loop of plain dispatches (C++ host)."""
+    import subprocess
+
+    from vk_compute_mipmaps_amd import build
+
+    cmd = [build.DEMO_PATH, "batch", "--count", "512", "--seed", "8", "--reps", "30", "--autotune"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(cmd)} failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    out = json.loads(r.stdout)
+    out["command"] = "vk_compute_mipmaps_amd/lib/nvpro_pyramid_demo " + " ".join(cmd[1:])
+    return out
+
+
+def large_configs(stream, reps=10):
+    """Single chains of the large configurations: median us and algorithmic GB/s."""
+    from vk_compute_mipmaps_amd import hip_rt, pyramid as P
+
+    res = {}
+    for name, w, h, fmt, arith, seed in (("16384x16384 sRGBA8", 16384, 16384, P.FORMAT_SRGBA8, P.ARITH_FP32, 6),
+                                         ("8192x8192 RGBA16F fp32", 8192, 8192, P.FORMAT_RGBA16F, P.ARITH_FP32, 7),
+                                         ("8192x8192 RGBA16F packed_fp16", 8192, 8192, P.FORMAT_RGBA16F,
+                                          P.ARITH_PACKED_FP16, 7)):
+        c = P.DeviceMipChain(w, h, fmt)
+        c.synthesize(seed, stream)
+        c.dispatch(stream, arithmetic=arith)
+        hip_rt.stream_sync(stream)
+        evs = [hip_rt.event_create() for _ in range(reps + 1)]
+        hip_rt.event_record(evs[0], stream)
+        for i in range(reps):
+            c.dispatch(stream, arithmetic=arith)
+            hip_rt.event_record(evs[i + 1], stream)
+        hip_rt.event_sync(evs[-1])
+        per = sorted(hip_rt.event_elapsed_ms(evs[i], evs[i + 1]) * 1000.0 for i in range(reps))
+        for e in evs:
+            hip_rt.event_destroy(e)
+        us = statistics.median(per)
+        res[name] = {"us": round(us, 2), "GBps": round(c.nbytes / (us * 1e-6) / 1e9, 1),
+                     "algorithmic_bytes": c.nbytes, "plan": " ".join(map(repr, P.plan(w, h, fmt=fmt)))}
+        c.free()
+    return res
 
 
 def cpu_baseline(level0, seconds):

@@ -249,6 +249,68 @@ int nvproPyramidBatchAutotune(const NvproPyramidPipelines* pipelines, const Nvpr
                               const uint32_t* mipLevels, uint32_t imageCount, hipStream_t stream,
                               uint32_t repetitions, NvproPyramidBatchConfig* best, float* bestMicroseconds);
 
+
+/* ------------------------------------------------------------------------- */
+/* Serving loop.  No reference counterpart (the reference regenerates one       */
+/* image's chain per frame, demo_app/mipmaps_app.cpp:362-369).  A server admits */
+/* mip-chain requests against an HBM budget, keeps each chain resident in its   */
+/* arena until released, and batches requests as they arrive: every poll       */
+/* launches the admitted requests (one batch per format, on a free in-flight    */
+/* slot with its own stream) while earlier batches are still running.  Each     */
+/* chain is planned as nvproPyramidDispatch plans it, so a result is the same   */
+/* whatever it was batched with.  All entry points are thread-safe.             */
+/* ------------------------------------------------------------------------- */
+
+typedef struct NvproPyramidServerConfig {
+  uint64_t budgetBytes;         /* device arena for resident chains; default 80 GB */
+  uint32_t maxInFlight;         /* batches on the device at once (1..16); default 4 */
+  uint32_t maxBatchImages;      /* images per batch; default 1024 */
+  uint32_t arithmetic16f;       /* NVPRO_PYRAMID_ARITH_* of RGBA16F requests; default FP32 */
+  uint32_t reserved[2];         /* must be 0 */
+  NvproPyramidBatchConfig batch; /* work-list configuration; useGraph must be 0 */
+} NvproPyramidServerConfig;
+
+typedef struct NvproPyramidRequest {
+  uint32_t format;     /* NVPRO_PYRAMID_FORMAT_SRGBA8 or _RGBA16F */
+  uint32_t width, height;
+  uint32_t mipLevels;  /* 0: full chain */
+  uint64_t seed;       /* level 0 is the seeded synthetic image (nvproPyramidSynthesize) ... */
+  const void* level0;  /* ... unless this points at tight host texels, copied at launch; they must
+                          stay valid until the request completes */
+} NvproPyramidRequest;
+
+typedef struct NvproPyramidCompletion {
+  uint64_t ticket;
+  int status;               /* 0 or an error */
+  uint32_t format;
+  NvproPyramidImage image;  /* the resident chain: packed layout in the server's arena */
+  uint64_t bytes;
+  double latencyUs;         /* submit to completion seen by a poll */
+  double waitUs;            /* submit to admission */
+} NvproPyramidCompletion;
+
+typedef struct NvproPyramidServerStats {
+  uint64_t submitted, admitted, completed, released, batches, batchedImages, failedBatches;
+  uint64_t budgetBytes, residentBytes, peakResidentBytes;
+  uint32_t waiting, inFlightBatches;
+} NvproPyramidServerStats;
+
+typedef struct NvproPyramidServer NvproPyramidServer;
+
+int nvproPyramidServerDefaultConfig(NvproPyramidServerConfig* config);
+/* Allocates the arena and the in-flight slots.  config NULL: defaults. */
+int nvproPyramidServerCreate(const NvproPyramidServerConfig* config, NvproPyramidServer** server);
+/* Queues a request; hipErrorOutOfMemory if its chain can never fit the budget. */
+int nvproPyramidServerSubmit(NvproPyramidServer* server, const NvproPyramidRequest* request, uint64_t* ticket);
+/* Makes progress (retire, admit, launch) and returns up to `capacity` completions. */
+int nvproPyramidServerPoll(NvproPyramidServer* server, NvproPyramidCompletion* completions, uint32_t capacity,
+                           uint32_t* count);
+/* Frees a completed request's resident chain. */
+int nvproPyramidServerRelease(NvproPyramidServer* server, uint64_t ticket);
+int nvproPyramidServerGetStats(NvproPyramidServer* server, NvproPyramidServerStats* stats);
+/* Waits for in-flight batches and frees everything. */
+int nvproPyramidServerDestroy(NvproPyramidServer* server);
+
 #ifdef __cplusplus
 }
 #endif

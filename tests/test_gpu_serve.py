@@ -1,0 +1,114 @@
+"""GPU tests of the serving loop (nvproPyramidServer*).
+
+"Alone == in the mix": a request's chain, computed in a batch with whatever
+else was admitted at the time, must equal byte for byte the chain of the same
+level 0 computed alone by nvproPyramidDispatch.  Admission: with a budget that
+holds only a few chains, requests wait and are admitted as earlier chains are
+released, resident bytes never exceed the budget, and a chain larger than the
+budget is refused at submission.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from vk_compute_mipmaps_amd import hip_rt
+from vk_compute_mipmaps_amd import pyramid as P
+from vk_compute_mipmaps_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _download(image, nbytes):
+    out = np.empty(nbytes, np.uint8)
+    hip_rt.memcpy_dtoh(out.ctypes.data, image.data, nbytes)
+    return out
+
+
+def _alone(fmt, w, h, seed, stream, arithmetic=P.ARITH_FP32, level0=None):
+    c = P.DeviceMipChain(w, h, fmt)
+    try:
+        if level0 is None:
+            c.synthesize(seed, stream)
+        else:
+            c.upload_level0(level0)
+        c.dispatch(stream=stream, arithmetic=arithmetic)
+        hip_rt.stream_sync(stream)
+        return c.download()
+    finally:
+        c.free()
+
+
+def _drain(srv, tickets, timeout=60.0):
+    done = {}
+    t_end = time.time() + timeout
+    while len(done) < len(tickets):
+        for c in srv.poll():
+            done[c.ticket] = c
+        assert time.time() < t_end, f"only {len(done)} of {len(tickets)} requests completed"
+    return done
+
+
+@pytest.mark.parametrize("arith", [P.ARITH_FP32, P.ARITH_PACKED_FP16])
+def test_alone_equals_in_mix(arith, stream):
+    sizes = synth.mixed_sizes(40, 21, hi=1024)
+    fmts = [P.FORMAT_RGBA16F if k % 3 == 1 else P.FORMAT_SRGBA8 for k in range(len(sizes))]
+    srv = P.PyramidServer(P.server_config(max_in_flight=3, max_batch_images=7, arithmetic16f=arith))
+    try:
+        tickets = {}
+        # Arrivals in waves, polled in between, so batches form from whatever has arrived.
+        for k, ((w, h), fmt) in enumerate(zip(sizes, fmts)):
+            level0 = synth.srgba8(w, h, 900 + k) if k % 5 == 0 and fmt == P.FORMAT_SRGBA8 else None
+            t = srv.submit(fmt, w, h, seed=900 + k, level0=level0)
+            tickets[t] = (k, w, h, fmt, level0)
+            if k % 4 == 3:
+                srv.poll(0)
+        done = _drain(srv, tickets)
+        st = srv.stats()
+        assert st.batches >= 6 and st.failedBatches == 0
+        for t, c in done.items():
+            k, w, h, fmt, level0 = tickets[t]
+            assert c.status == 0
+            got = _download(c.image, c.bytes)
+            want = _alone(fmt, w, h, 900 + k, stream, arithmetic=arith if fmt == P.FORMAT_RGBA16F else P.ARITH_FP32,
+                          level0=level0)
+            assert np.array_equal(got, want), f"request {k} ({w}x{h}, format {fmt}) differs from its alone result"
+            srv.release(t)
+        assert srv.stats().residentBytes == 0
+    finally:
+        srv.destroy()
+
+
+def test_admission_by_budget(stream):
+    w = h = 1024
+    chain = P.chain_bytes(w, h)
+    budget = 3 * ((chain + 255) // 256 * 256)  # room for three chains
+    srv = P.PyramidServer(P.server_config(budget_bytes=budget, max_in_flight=2))
+    try:
+        with pytest.raises(P.NvproPyramidError):
+            srv.submit(P.FORMAT_SRGBA8, 2048, 2048, seed=1)  # can never fit
+        tickets = [srv.submit(P.FORMAT_SRGBA8, w, h, seed=50 + k) for k in range(10)]
+        done, released = {}, 0
+        t_end = time.time() + 60
+        while released < len(tickets):
+            for c in srv.poll():
+                done[c.ticket] = c
+            st = srv.stats()
+            assert st.residentBytes <= budget and st.peakResidentBytes <= budget
+            assert st.admitted - st.released <= 3
+            # The client consumes completed chains one at a time, in completion order.
+            pending = [t for t in tickets if t in done and done[t] is not None]
+            if pending:
+                t = pending[0]
+                got = _download(done[t].image, done[t].bytes)
+                k = tickets.index(t)
+                assert np.array_equal(got, _alone(P.FORMAT_SRGBA8, w, h, 50 + k, stream))
+                srv.release(t)
+                done[t] = None
+                released += 1
+            assert time.time() < t_end
+        st = srv.stats()
+        assert st.completed == 10 and st.released == 10 and st.residentBytes == 0
+        assert st.peakResidentBytes == budget
+    finally:
+        srv.destroy()

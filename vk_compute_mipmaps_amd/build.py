@@ -30,6 +30,7 @@ SOURCES = [
     "synth.hip",
     "nvpro_pyramid_dispatch.cpp",
     "nvpro_pyramid_batch.cpp",
+    "nvpro_pyramid_server.cpp",
 ]
 
 HIPCC_FLAGS = [

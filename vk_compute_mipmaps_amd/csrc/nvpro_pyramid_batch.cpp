@@ -16,6 +16,7 @@
 #include <tuple>
 #include <vector>
 
+#include "batch_internal.hpp"
 #include "groups.hpp"
 
 using namespace nvpro_pyramid;
@@ -24,17 +25,6 @@ namespace {
 
 constexpr uint32_t kDefaultItemBytes = 65536;
 constexpr uint32_t kSpinLimit        = 1u << 20;  // polls of >= 50 ns: far beyond any legitimate wait
-
-struct Plan
-{
-  std::vector<ImageDesc> imgs;
-  std::vector<QueueSeg> segs;
-  std::vector<QueueItem> items;
-  std::vector<TailPlan> tails;
-  std::vector<uint32_t> roundEnd;  // items [roundEnd[r-1], roundEnd[r]) form round r
-  uint32_t numSlots        = 0;
-  uint64_t algorithmicBytes = 0;
-};
 
 // Input bytes one group of a segment reads (for item sizing).
 uint64_t groupInputBytes(uint32_t kind, uint32_t bpt)
@@ -50,7 +40,11 @@ uint64_t groupInputBytes(uint32_t kind, uint32_t bpt)
   return 1u;
 }
 
-int checkConfig(const NvproPyramidBatchConfig& c, const char* who)
+size_t align256(size_t v) { return (v + 255u) & ~size_t(255u); }
+
+}  // namespace
+
+int nvpro_pyramid::checkBatchConfig(const NvproPyramidBatchConfig& c, const char* who)
 {
   if(c.mode > NVPRO_PYRAMID_BATCH_MODE_QUEUE)
     return hostFail(hipErrorInvalidValue, "%s: unknown mode %u", who, c.mode);
@@ -61,14 +55,15 @@ int checkConfig(const NvproPyramidBatchConfig& c, const char* who)
   return 0;
 }
 
-int buildPlan(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* images, const uint32_t* mipLevels,
-              uint32_t count, const NvproPyramidBatchConfig& cfg, Plan& plan, const char* who)
+int nvpro_pyramid::buildBatchPlan(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* images,
+                                  const uint32_t* mipLevels, uint32_t count, const NvproPyramidBatchConfig& cfg,
+                                  BatchPlan& plan, const char* who)
 {
   if(!images || count == 0)
     return hostFail(hipErrorInvalidValue, "%s: no images", who);
   const uint32_t bpt       = pipelines ? nvproPyramidBytesPerTexel(pipelines->format) : 0u;
   const uint32_t itemBytes = cfg.itemBytes ? cfg.itemBytes : kDefaultItemBytes;
-  plan                     = Plan{};
+  plan                     = BatchPlan{};
   plan.imgs.resize(count);
   // Per image: its steps as segments (before item cutting) in step order.
   struct Pending
@@ -176,54 +171,138 @@ int buildPlan(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* i
   return 0;
 }
 
-size_t align256(size_t v) { return (v + 255u) & ~size_t(255u); }
-
-}  // namespace
-
-struct NvproPyramidBatch
+void nvpro_pyramid::defaultBatchConfig(NvproPyramidBatchConfig* c)
 {
-  NvproPyramidPipelines pipelines{};
-  NvproPyramidBatchConfig cfg{};
-  Plan plan;
-  void* dev          = nullptr;
-  size_t devBytes    = 0;
-  BatchArgs args{};
-  uint32_t blocks    = 0;  // workgroups of a queue launch / cap of a round launch
-  uint32_t launches  = 0;
-  uint32_t maxBlocks = 0;
-  hipStream_t priv   = nullptr;
-  hipGraph_t graph   = nullptr;
-  hipGraphExec_t exec = nullptr;
-};
+  memset(c, 0, sizeof *c);
+  c->mode            = NVPRO_PYRAMID_BATCH_MODE_QUEUE;
+  c->workgroupsPerCU = 0;
+  c->itemBytes       = kDefaultItemBytes;
+  c->tailTexels      = kBatchTailTexels;
+  c->useGraph        = 1;
+}
 
-static hipError_t enqueueBatch(NvproPyramidBatch* b, hipStream_t stream)
+// ---------------------------------------------------------------------------
+// BatchRunner
+// ---------------------------------------------------------------------------
+
+BatchRunner::~BatchRunner()
 {
-  const bool queue = b->cfg.mode == NVPRO_PYRAMID_BATCH_MODE_QUEUE;
-  if(b->plan.items.empty())
-    return hipSuccess;
-  if(queue)
+  if(dev_)
+    (void)hipFree(dev_);
+  if(pinned_)
+    (void)hipHostFree(pinned_);
+}
+
+hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& pipes,
+                              const NvproPyramidBatchConfig& cfg, hipStream_t stream)
+{
+  // Device layout: images | segments | items | tails | counters (zeroed).
+  const size_t oImg = 0;
+  const size_t oSeg = align256(oImg + p.imgs.size() * sizeof(ImageDesc));
+  const size_t oItm = align256(oSeg + p.segs.size() * sizeof(QueueSeg));
+  const size_t oTl  = align256(oItm + p.items.size() * sizeof(QueueItem));
+  const size_t oCnt = align256(oTl + p.tails.size() * sizeof(TailPlan));
+  const size_t size = align256(oCnt + (kCounterSlots + p.numSlots) * sizeof(uint32_t));
+  if(size > devCap_)
   {
-    BatchArgs a  = b->args;
+    // Grow (the previous buffer is idle: the caller staged nothing in flight).
+    if(dev_)
+      (void)hipFree(dev_);
+    dev_    = nullptr;
+    devCap_ = 0;
+    if(hipError_t e = hipMalloc(&dev_, size))
+      return e;
+    devCap_ = size;
+  }
+  if(size > pinnedCap_)
+  {
+    if(pinned_)
+      (void)hipHostFree(pinned_);
+    pinned_    = nullptr;
+    pinnedCap_ = 0;
+    if(hipError_t e = hipHostMalloc(&pinned_, size, hipHostMallocDefault))
+      return e;
+    pinnedCap_ = size;
+  }
+  uint8_t* h = (uint8_t*)pinned_;
+  memset(h, 0, size);
+  memcpy(h + oImg, p.imgs.data(), p.imgs.size() * sizeof(ImageDesc));
+  if(!p.segs.empty())
+    memcpy(h + oSeg, p.segs.data(), p.segs.size() * sizeof(QueueSeg));
+  if(!p.items.empty())
+    memcpy(h + oItm, p.items.data(), p.items.size() * sizeof(QueueItem));
+  if(!p.tails.empty())
+    memcpy(h + oTl, p.tails.data(), p.tails.size() * sizeof(TailPlan));
+  if(hipError_t e = hipMemcpyAsync(dev_, pinned_, size, hipMemcpyHostToDevice, stream))
+    return e;
+  uint8_t* base   = (uint8_t*)dev_;
+  args_.imgs      = (const ImageDesc*)(base + oImg);
+  args_.segs      = (const QueueSeg*)(base + oSeg);
+  args_.items     = (const QueueItem*)(base + oItm);
+  args_.tails     = (const TailPlan*)(base + oTl);
+  args_.counters  = (uint32_t*)(base + oCnt);
+  args_.numSlots  = p.numSlots;
+  args_.spinLimit = kSpinLimit;
+  format_         = pipes.format;
+  arithmetic_     = pipes.arithmetic;
+  queue_          = cfg.mode == NVPRO_PYRAMID_BATCH_MODE_QUEUE;
+  roundEnd_       = p.roundEnd;
+  items_          = (uint32_t)p.items.size();
+  const uint32_t fit   = batchMaxWorkgroupsPerCU(format_, arithmetic_, queue_);
+  const uint32_t perCU = cfg.workgroupsPerCU ? std::min(cfg.workgroupsPerCU, fit) : fit;
+  blocks               = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>((uint64_t)deviceComputeUnits() * perCU,
+                                                                                items_));
+  launches             = 0;
+  uint32_t begin       = 0;
+  for(uint32_t end : roundEnd_)
+  {
+    launches += end > begin ? 1u : 0u;
+    begin = end;
+  }
+  if(queue_ && items_ != 0)
+    launches = 1u;
+  devBytes = size;
+  return hipSuccess;
+}
+
+hipError_t BatchRunner::launch(hipStream_t stream) const
+{
+  if(items_ == 0)
+    return hipSuccess;
+  if(queue_)
+  {
+    BatchArgs a  = args_;
     a.itemBegin = 0;
-    a.itemEnd   = (uint32_t)b->plan.items.size();
-    return launchBatch(stream, b->pipelines.format, b->pipelines.arithmetic, true, a, b->blocks);
+    a.itemEnd   = items_;
+    return launchBatch(stream, format_, arithmetic_, true, a, blocks);
   }
   uint32_t begin = 0;
-  for(uint32_t end : b->plan.roundEnd)
+  for(uint32_t end : roundEnd_)
   {
     if(end > begin)
     {
-      BatchArgs a  = b->args;
+      BatchArgs a  = args_;
       a.itemBegin = begin;
       a.itemEnd   = end;
-      const uint32_t n = std::min(end - begin, b->blocks);
-      if(hipError_t e = launchBatch(stream, b->pipelines.format, b->pipelines.arithmetic, false, a, n))
+      if(hipError_t e = launchBatch(stream, format_, arithmetic_, false, a, std::min(end - begin, blocks)))
         return e;
     }
     begin = end;
   }
   return hipSuccess;
 }
+
+uint32_t* BatchRunner::errorWord() const { return args_.counters ? args_.counters + kCounterError : nullptr; }
+
+struct NvproPyramidBatch
+{
+  NvproPyramidBatchConfig cfg{};
+  BatchPlan plan;
+  BatchRunner runner;
+  hipStream_t priv    = nullptr;
+  hipGraph_t graph    = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
 
 static void releaseBatch(NvproPyramidBatch* b)
 {
@@ -232,11 +311,10 @@ static void releaseBatch(NvproPyramidBatch* b)
   if(b->graph)
     (void)hipGraphDestroy(b->graph);
   if(b->priv)
+  {
     (void)hipStreamSynchronize(b->priv);
-  if(b->dev)
-    (void)hipFree(b->dev);
-  if(b->priv)
     (void)hipStreamDestroy(b->priv);
+  }
   delete b;
 }
 
@@ -246,12 +324,7 @@ int nvproPyramidBatchDefaultConfig(NvproPyramidBatchConfig* config)
 {
   if(!config)
     return hostFail(hipErrorInvalidValue, "nvproPyramidBatchDefaultConfig: null config");
-  memset(config, 0, sizeof *config);
-  config->mode            = NVPRO_PYRAMID_BATCH_MODE_QUEUE;
-  config->workgroupsPerCU = 0;
-  config->itemBytes       = kDefaultItemBytes;
-  config->tailTexels      = kBatchTailTexels;
-  config->useGraph        = 1;
+  defaultBatchConfig(config);
   return 0;
 }
 
@@ -264,40 +337,21 @@ int nvproPyramidBatchCreate(const NvproPyramidPipelines* pipelines, const NvproP
     return hostFail(hipErrorInvalidValue, "%s: null output", who);
   *batch = nullptr;
   NvproPyramidBatchConfig cfg;
-  nvproPyramidBatchDefaultConfig(&cfg);
+  defaultBatchConfig(&cfg);
   if(config)
     cfg = *config;
-  if(int e = checkConfig(cfg, who))
+  if(int e = checkBatchConfig(cfg, who))
     return e;
   NvproPyramidBatch* b = new NvproPyramidBatch;
   b->cfg               = cfg;
-  b->pipelines         = pipelines ? *pipelines : NvproPyramidPipelines{};
-  if(int e = buildPlan(pipelines, images, mipLevels, imageCount, cfg, b->plan, who))
+  if(int e = buildBatchPlan(pipelines, images, mipLevels, imageCount, cfg, b->plan, who))
   {
     delete b;
     return e;
   }
-  const Plan& p = b->plan;
-  // Device layout: images | segments | items | tails | counters.
-  const size_t oImg = 0;
-  const size_t oSeg = align256(oImg + p.imgs.size() * sizeof(ImageDesc));
-  const size_t oItm = align256(oSeg + p.segs.size() * sizeof(QueueSeg));
-  const size_t oTl  = align256(oItm + p.items.size() * sizeof(QueueItem));
-  const size_t oCnt = align256(oTl + p.tails.size() * sizeof(TailPlan));
-  b->devBytes       = align256(oCnt + (kCounterSlots + p.numSlots) * sizeof(uint32_t));
-  std::vector<uint8_t> host(b->devBytes, 0);
-  memcpy(host.data() + oImg, p.imgs.data(), p.imgs.size() * sizeof(ImageDesc));
-  if(!p.segs.empty())
-    memcpy(host.data() + oSeg, p.segs.data(), p.segs.size() * sizeof(QueueSeg));
-  if(!p.items.empty())
-    memcpy(host.data() + oItm, p.items.data(), p.items.size() * sizeof(QueueItem));
-  if(!p.tails.empty())
-    memcpy(host.data() + oTl, p.tails.data(), p.tails.size() * sizeof(TailPlan));
   hipError_t e = hipStreamCreateWithFlags(&b->priv, hipStreamNonBlocking);
   if(e == hipSuccess)
-    e = hipMalloc(&b->dev, b->devBytes);
-  if(e == hipSuccess)
-    e = hipMemcpyAsync(b->dev, host.data(), b->devBytes, hipMemcpyHostToDevice, b->priv);
+    e = b->runner.stage(b->plan, *pipelines, cfg, b->priv);
   if(e == hipSuccess)
     e = hipStreamSynchronize(b->priv);
   if(e != hipSuccess)
@@ -305,35 +359,12 @@ int nvproPyramidBatchCreate(const NvproPyramidPipelines* pipelines, const NvproP
     releaseBatch(b);
     return hostFail(e, "%s: device work list: %s", who, hipGetErrorString(e));
   }
-  uint8_t* base    = (uint8_t*)b->dev;
-  b->args.imgs     = (const ImageDesc*)(base + oImg);
-  b->args.segs     = (const QueueSeg*)(base + oSeg);
-  b->args.items    = (const QueueItem*)(base + oItm);
-  b->args.tails    = (const TailPlan*)(base + oTl);
-  b->args.counters = (uint32_t*)(base + oCnt);
-  b->args.numSlots = p.numSlots;
-  b->args.spinLimit = kSpinLimit;
-  const bool queue     = cfg.mode == NVPRO_PYRAMID_BATCH_MODE_QUEUE;
-  const uint32_t fit   = batchMaxWorkgroupsPerCU(b->pipelines.format, b->pipelines.arithmetic, queue);
-  const uint32_t perCU = cfg.workgroupsPerCU ? std::min(cfg.workgroupsPerCU, fit) : fit;
-  const uint64_t cap   = (uint64_t)deviceComputeUnits() * perCU;
-  b->blocks            = (uint32_t)std::max<uint64_t>(1u, std::min<uint64_t>(cap, p.items.size()));
-  b->launches          = 0;
-  uint32_t begin       = 0;
-  for(uint32_t end : p.roundEnd)
-  {
-    if(end > begin)
-      b->launches += 1u;
-    begin = end;
-  }
-  if(queue && !p.items.empty())
-    b->launches = 1u;
-  if(cfg.useGraph && !p.items.empty())
+  if(cfg.useGraph && !b->plan.items.empty())
   {
     e = hipStreamBeginCapture(b->priv, hipStreamCaptureModeThreadLocal);
     if(e == hipSuccess)
     {
-      hipError_t le = enqueueBatch(b, b->priv);
+      hipError_t le = b->runner.launch(b->priv);
       e             = hipStreamEndCapture(b->priv, &b->graph);
       if(le != hipSuccess)
         e = le;
@@ -355,7 +386,7 @@ int nvproPyramidBatchLaunch(NvproPyramidBatch* batch, hipStream_t stream)
 {
   if(!batch)
     return hostFail(hipErrorInvalidValue, "nvproPyramidBatchLaunch: null batch");
-  hipError_t e = batch->exec ? hipGraphLaunch(batch->exec, stream) : enqueueBatch(batch, stream);
+  hipError_t e = batch->exec ? hipGraphLaunch(batch->exec, stream) : batch->runner.launch(stream);
   if(e != hipSuccess)
     return hostFail(e, "nvproPyramidBatchLaunch: %s", hipGetErrorString(e));
   return 0;
@@ -370,10 +401,10 @@ int nvproPyramidBatchGetInfo(const NvproPyramidBatch* batch, NvproPyramidBatchIn
   info->segmentCount     = (uint32_t)batch->plan.segs.size();
   info->itemCount        = (uint32_t)batch->plan.items.size();
   info->roundCount       = (uint32_t)batch->plan.roundEnd.size();
-  info->launchCount      = batch->launches;
-  info->workgroups       = batch->plan.items.empty() ? 0u : batch->blocks;
+  info->launchCount      = batch->runner.launches;
+  info->workgroups       = batch->plan.items.empty() ? 0u : batch->runner.blocks;
   info->algorithmicBytes = batch->plan.algorithmicBytes;
-  info->deviceBytes      = batch->devBytes;
+  info->deviceBytes      = batch->runner.devBytes;
   info->config           = batch->cfg;
   return 0;
 }
@@ -384,7 +415,7 @@ int nvproPyramidBatchReadErrors(NvproPyramidBatch* batch, uint32_t* flags)
     return hostFail(hipErrorInvalidValue, "nvproPyramidBatchReadErrors: null argument");
   hipError_t e = hipDeviceSynchronize();
   uint32_t v   = 0;
-  uint32_t* p  = batch->args.counters + kCounterError;
+  uint32_t* p  = batch->runner.errorWord();
   if(e == hipSuccess)
     e = hipMemcpyAsync(&v, p, sizeof v, hipMemcpyDeviceToHost, batch->priv);
   const uint32_t zero = 0;

@@ -13,6 +13,14 @@
 //       Times a loop of plain per-texture dispatches (eager, and captured into
 //       one hipGraph like a recorded command buffer) against one batched launch,
 //       checks that both write identical bytes, prints one JSON object.
+//   nvpro_pyramid_demo serve [--rate R] [--count N] [--seed S] [--hold-ms H]
+//                            [--budget-gb B] [--in-flight K] [--packed]
+//       Serving loop: N requests arrive as a Poisson stream of R per second;
+//       sizes from the mixed-size workload, 70% sRGBA8 / 30% RGBA16F; level 0
+//       is the seeded synthetic image, written on the device at launch.  The
+//       server admits against a B GB budget and batches what arrived; each
+//       completed chain stays resident for H ms before the client releases it.
+//       Prints chains/s, p50/p99 latency and batching statistics as JSON.
 //   nvpro_pyramid_demo sizes N S
 //       Prints the workload's N texture sizes for seed S (no device use).
 #include <hip/hip_runtime.h>
@@ -22,6 +30,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -274,8 +283,130 @@ static int batch(const Args& a)
   return identical && flags == 0 ? 0 : 1;
 }
 
+struct ServeArgs
+{
+  double rate = 20000.0, holdMs = 20.0, budgetGB = 80.0;
+  uint32_t count = 20000, inFlight = 4;
+  uint64_t seed = 9;
+  bool packed = false;
+};
+
+static int serve(const ServeArgs& a)
+{
+  using Clock        = std::chrono::steady_clock;
+  const auto sizes   = mixedSizes(a.count, a.seed);
+  std::vector<double> arrival(a.count);
+  std::vector<uint32_t> format(a.count);
+  double t = 0;
+  for(uint32_t k = 0; k < a.count; ++k)
+  {
+    const double u = ((double)(hash4(a.seed, 3, k, k) >> 11) + 0.5) * 0x1p-53;
+    t += -log(u) / a.rate;  // exponential inter-arrival times
+    arrival[k] = t;
+    const double f = (double)(hash4(a.seed, 4, k, k) >> 11) * 0x1p-53;
+    format[k]      = f < 0.7 ? NVPRO_PYRAMID_FORMAT_SRGBA8 : NVPRO_PYRAMID_FORMAT_RGBA16F;
+  }
+  NvproPyramidServerConfig cfg;
+  CHECK_PYR(nvproPyramidServerDefaultConfig(&cfg));
+  cfg.budgetBytes   = (uint64_t)(a.budgetGB * 1e9);
+  cfg.maxInFlight   = a.inFlight;
+  cfg.arithmetic16f = a.packed ? NVPRO_PYRAMID_ARITH_PACKED_FP16 : NVPRO_PYRAMID_ARITH_FP32;
+  NvproPyramidServer* srv = nullptr;
+  CHECK_PYR(nvproPyramidServerCreate(&cfg, &srv));
+  std::vector<NvproPyramidCompletion> done(4096);
+  std::vector<double> latency, wait;
+  std::vector<std::pair<double, uint64_t>> holding;  // (release time, ticket)
+  uint64_t bytes = 0;
+  uint32_t next = 0, finished = 0, failed = 0;
+  const Clock::time_point t0 = Clock::now();
+  double lastDone            = 0;
+  while(finished < a.count)
+  {
+    const double now = std::chrono::duration<double>(Clock::now() - t0).count();
+    while(next < a.count && arrival[next] <= now)
+    {
+      NvproPyramidRequest r{format[next], sizes[next].first, sizes[next].second, 0, a.seed * 100000u + next,
+                            nullptr};
+      uint64_t ticket;
+      CHECK_PYR(nvproPyramidServerSubmit(srv, &r, &ticket));
+      ++next;
+    }
+    uint32_t n = 0;
+    CHECK_PYR(nvproPyramidServerPoll(srv, done.data(), (uint32_t)done.size(), &n));
+    for(uint32_t i = 0; i < n; ++i)
+    {
+      latency.push_back(done[i].latencyUs);
+      wait.push_back(done[i].waitUs);
+      bytes += done[i].bytes;
+      failed += done[i].status != 0 ? 1u : 0u;
+      holding.push_back({now + a.holdMs * 1e-3, done[i].ticket});
+      ++finished;
+      lastDone = now;
+    }
+    size_t keep = 0;
+    for(size_t i = 0; i < holding.size(); ++i)
+    {
+      if(holding[i].first <= now)
+        CHECK_PYR(nvproPyramidServerRelease(srv, holding[i].second));
+      else
+        holding[keep++] = holding[i];
+    }
+    holding.resize(keep);
+  }
+  NvproPyramidServerStats st;
+  CHECK_PYR(nvproPyramidServerGetStats(srv, &st));
+  for(const auto& h : holding)
+    CHECK_PYR(nvproPyramidServerRelease(srv, h.second));
+  CHECK_PYR(nvproPyramidServerDestroy(srv));
+  const double span = lastDone;
+  printf("{\"requests\": %u, \"rate_per_s\": %.1f, \"mix\": \"70%% sRGBA8 / 30%% RGBA16F (%s)\", "
+         "\"budget_bytes\": %llu, \"in_flight\": %u, \"hold_ms\": %.1f,\n",
+         a.count, a.rate, a.packed ? "packed_fp16" : "fp32", (unsigned long long)cfg.budgetBytes, a.inFlight,
+         a.holdMs);
+  printf(" \"seconds\": %.4f, \"chains_per_s\": %.1f, \"GBps\": %.1f, \"failed\": %u,\n", span,
+         a.count / span, bytes / span / 1e9, failed);
+  printf(" \"latency_us\": {\"p50\": %.1f, \"p90\": %.1f, \"p99\": %.1f, \"max\": %.1f}, "
+         "\"admission_wait_us\": {\"p50\": %.1f, \"p99\": %.1f},\n",
+         percentile(latency, 0.5), percentile(latency, 0.9), percentile(latency, 0.99), percentile(latency, 1.0),
+         percentile(wait, 0.5), percentile(wait, 0.99));
+  printf(" \"batches\": %llu, \"mean_batch_images\": %.2f, \"peak_resident_bytes\": %llu}\n",
+         (unsigned long long)st.batches, st.batches ? (double)st.batchedImages / st.batches : 0.0,
+         (unsigned long long)st.peakResidentBytes);
+  return failed == 0 ? 0 : 1;
+}
+
 int main(int argc, char** argv)
 {
+  if(argc > 1 && !strcmp(argv[1], "serve"))
+  {
+    ServeArgs a;
+    for(int i = 2; i < argc; ++i)
+    {
+      const std::string o = argv[i];
+      if(o == "--rate" && i + 1 < argc)
+        a.rate = atof(argv[++i]);
+      else if(o == "--count" && i + 1 < argc)
+        a.count = (uint32_t)atoi(argv[++i]);
+      else if(o == "--seed" && i + 1 < argc)
+        a.seed = (uint64_t)atoll(argv[++i]);
+      else if(o == "--hold-ms" && i + 1 < argc)
+        a.holdMs = atof(argv[++i]);
+      else if(o == "--budget-gb" && i + 1 < argc)
+        a.budgetGB = atof(argv[++i]);
+      else if(o == "--in-flight" && i + 1 < argc)
+        a.inFlight = (uint32_t)atoi(argv[++i]);
+      else if(o == "--packed")
+        a.packed = true;
+      else
+      {
+        fprintf(stderr, "unknown option %s\n", o.c_str());
+        return 4;
+      }
+    }
+    if(a.count == 0 || !(a.rate > 0))
+      return 4;
+    return serve(a);
+  }
   if(argc > 3 && !strcmp(argv[1], "sizes"))  // host only: the workload's texture sizes
   {
     for(const auto& wh : mixedSizes((uint32_t)atoi(argv[2]), (uint64_t)atoll(argv[3])))

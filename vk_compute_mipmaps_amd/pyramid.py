@@ -52,6 +52,13 @@ EXPORTED_SYMBOLS = (
     "nvproPyramidBatchReadErrors",
     "nvproPyramidBatchDestroy",
     "nvproPyramidBatchAutotune",
+    "nvproPyramidServerDefaultConfig",
+    "nvproPyramidServerCreate",
+    "nvproPyramidServerSubmit",
+    "nvproPyramidServerPoll",
+    "nvproPyramidServerRelease",
+    "nvproPyramidServerGetStats",
+    "nvproPyramidServerDestroy",
 )
 
 BATCH_MODE_ROUNDS = 0
@@ -115,6 +122,33 @@ class NvproPyramidBatchInfo(ctypes.Structure):
                 ("config", NvproPyramidBatchConfig)]
 
 
+class NvproPyramidServerConfig(ctypes.Structure):
+    _fields_ = [("budgetBytes", ctypes.c_uint64), ("maxInFlight", ctypes.c_uint32),
+                ("maxBatchImages", ctypes.c_uint32), ("arithmetic16f", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2), ("batch", NvproPyramidBatchConfig)]
+
+
+class NvproPyramidRequest(ctypes.Structure):
+    _fields_ = [("format", ctypes.c_uint32), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("mipLevels", ctypes.c_uint32), ("seed", ctypes.c_uint64), ("level0", ctypes.c_void_p)]
+
+
+class NvproPyramidCompletion(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("status", ctypes.c_int), ("format", ctypes.c_uint32),
+                ("image", NvproPyramidImage), ("bytes", ctypes.c_uint64), ("latencyUs", ctypes.c_double),
+                ("waitUs", ctypes.c_double)]
+
+
+class NvproPyramidServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "submitted", "admitted", "completed", "released", "batches", "batchedImages", "failedBatches",
+        "budgetBytes", "residentBytes", "peakResidentBytes")] + [
+        ("waiting", ctypes.c_uint32), ("inFlightBatches", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 DISPATCHER = ctypes.CFUNCTYPE(ctypes.c_uint32, ctypes.c_void_p, ctypes.POINTER(NvproPyramidState))
 
 _lib = None
@@ -167,6 +201,17 @@ def lib():
                 ("nvproPyramidBatchAutotune", [ctypes.POINTER(NvproPyramidPipelines),
                                                ctypes.POINTER(NvproPyramidImage), ctypes.POINTER(u32), u32, vp, u32,
                                                cfgp, ctypes.POINTER(ctypes.c_float)])):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = args
+        for name, args in (
+                ("nvproPyramidServerDefaultConfig", [ctypes.POINTER(NvproPyramidServerConfig)]),
+                ("nvproPyramidServerCreate", [ctypes.POINTER(NvproPyramidServerConfig), ctypes.POINTER(vp)]),
+                ("nvproPyramidServerSubmit", [vp, ctypes.POINTER(NvproPyramidRequest),
+                                              ctypes.POINTER(ctypes.c_uint64)]),
+                ("nvproPyramidServerPoll", [vp, ctypes.POINTER(NvproPyramidCompletion), u32, ctypes.POINTER(u32)]),
+                ("nvproPyramidServerRelease", [vp, ctypes.c_uint64]),
+                ("nvproPyramidServerGetStats", [vp, ctypes.POINTER(NvproPyramidServerStats)]),
+                ("nvproPyramidServerDestroy", [vp])):
             getattr(L, name).restype = ctypes.c_int
             getattr(L, name).argtypes = args
         _lib = L
@@ -415,3 +460,71 @@ def batch_autotune(pipes: NvproPyramidPipelines, images, mip_levels=None, stream
                                            len(images), ctypes.c_void_p(stream), repetitions, ctypes.byref(best),
                                            ctypes.byref(us)))
     return best, us.value
+
+
+def server_config(budget_bytes: int = 0, max_in_flight: int = 0, max_batch_images: int = 0,
+                  arithmetic16f: int = ARITH_FP32, batch: NvproPyramidBatchConfig | None = None):
+    c = NvproPyramidServerConfig()
+    _check(lib().nvproPyramidServerDefaultConfig(ctypes.byref(c)))
+    if budget_bytes:
+        c.budgetBytes = budget_bytes
+    if max_in_flight:
+        c.maxInFlight = max_in_flight
+    if max_batch_images:
+        c.maxBatchImages = max_batch_images
+    c.arithmetic16f = arithmetic16f
+    if batch is not None:
+        c.batch = batch
+    return c
+
+
+class PyramidServer:
+    """nvproPyramidServer*: admission by HBM budget, resident chains, continuous batching."""
+
+    def __init__(self, config: NvproPyramidServerConfig | None = None):
+        h = ctypes.c_void_p()
+        _check(lib().nvproPyramidServerCreate(ctypes.byref(config) if config is not None else None,
+                                              ctypes.byref(h)))
+        self.handle = h.value
+        self._host = {}  # ticket -> host level 0 kept alive until completion
+
+    def submit(self, fmt: int, width: int, height: int, mip_levels: int = 0, seed: int = 0, level0=None) -> int:
+        r = NvproPyramidRequest(fmt, width, height, mip_levels, seed, None)
+        if level0 is not None:
+            import numpy as np
+
+            level0 = np.ascontiguousarray(level0)
+            r.level0 = level0.ctypes.data
+        t = ctypes.c_uint64()
+        _check(lib().nvproPyramidServerSubmit(ctypes.c_void_p(self.handle), ctypes.byref(r), ctypes.byref(t)))
+        if level0 is not None:
+            self._host[t.value] = level0
+        return t.value
+
+    def poll(self, capacity: int = 256):
+        arr = (NvproPyramidCompletion * capacity)()
+        n = ctypes.c_uint32()
+        _check(lib().nvproPyramidServerPoll(ctypes.c_void_p(self.handle), arr, capacity, ctypes.byref(n)))
+        out = [arr[i] for i in range(n.value)]
+        for c in out:
+            self._host.pop(c.ticket, None)
+        return out
+
+    def release(self, ticket: int):
+        _check(lib().nvproPyramidServerRelease(ctypes.c_void_p(self.handle), ticket))
+
+    def stats(self) -> NvproPyramidServerStats:
+        st = NvproPyramidServerStats()
+        _check(lib().nvproPyramidServerGetStats(ctypes.c_void_p(self.handle), ctypes.byref(st)))
+        return st
+
+    def destroy(self):
+        if self.handle:
+            lib().nvproPyramidServerDestroy(ctypes.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
