@@ -14,7 +14,8 @@ the host as a baseline.  Beside it:
   batched   512 mixed-size sRGBA8 textures (seed 8): one batched launch against a
             loop of plain per-texture dispatches, timed by the C++ host
             (lib/nvpro_pyramid_demo batch), chains/s and p50/p99 per batch;
-  configs   single chains of the large configurations (16384^2 sRGBA8, 8192^2
+  configs   single chains of the other configurations (1024^2; the odd sizes
+            4095x2047, 1920x1080, 2047x4097 and 4095^2; 16384^2 sRGBA8; 8192^2
             RGBA16F with FP32 and packed-FP16 arithmetic): us and GB/s.
 """
 from __future__ import annotations
@@ -145,11 +146,13 @@ def main():
     hip_rt.stream_destroy(stream)
     batched = None if args.no_batched else batched_leg()
 
-    traffic = None
-    tpath = os.path.join(REPO, "profiles", "fast_pipeline_4096_traffic.json")
+    traffic = None  # HBM bytes per F(0,6) launch, from the rocprofv3 --pmc summary
+    tpath = os.path.join(REPO, "profiles", "fast_pipeline_srgba8_4096.md")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            for line in f:
+                if line.startswith("- traffic_bytes_per_launch:"):
+                    traffic = int(line.split(":")[1].split()[0])
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -216,12 +219,17 @@ def batched_leg():
     return out
 
 
-def large_configs(stream, reps=10):
-    """Single chains of the large configurations: median us and algorithmic GB/s."""
+def large_configs(stream, reps=20):
+    """Single chains of the other configurations: median us and algorithmic GB/s."""
     from vk_compute_mipmaps_amd import hip_rt, pyramid as P
 
     res = {}
-    for name, w, h, fmt, arith, seed in (("16384x16384 sRGBA8", 16384, 16384, P.FORMAT_SRGBA8, P.ARITH_FP32, 6),
+    for name, w, h, fmt, arith, seed in (("1024x1024 sRGBA8", 1024, 1024, P.FORMAT_SRGBA8, P.ARITH_FP32, 1),
+                                         ("4095x2047 sRGBA8", 4095, 2047, P.FORMAT_SRGBA8, P.ARITH_FP32, 3),
+                                         ("1920x1080 sRGBA8", 1920, 1080, P.FORMAT_SRGBA8, P.ARITH_FP32, 4),
+                                         ("2047x4097 sRGBA8", 2047, 4097, P.FORMAT_SRGBA8, P.ARITH_FP32, 5),
+                                         ("4095x4095 sRGBA8", 4095, 4095, P.FORMAT_SRGBA8, P.ARITH_FP32, 11),
+                                         ("16384x16384 sRGBA8", 16384, 16384, P.FORMAT_SRGBA8, P.ARITH_FP32, 6),
                                          ("8192x8192 RGBA16F fp32", 8192, 8192, P.FORMAT_RGBA16F, P.ARITH_FP32, 7),
                                          ("8192x8192 RGBA16F packed_fp16", 8192, 8192, P.FORMAT_RGBA16F,
                                           P.ARITH_PACKED_FP16, 7)):

@@ -32,7 +32,7 @@ struct BatchScratch
 {
   using V                     = typename F::Val;
   static constexpr size_t a   = sizeof(V) * kGroupBS;                            // fast: level2
-  static constexpr size_t b   = sizeof(V) * kGeneralMidEdge * kGeneralMidEdge;  // general: mid
+  static constexpr size_t b   = sizeof(V) * gen2MidTexels<F>();                 // general: mid
   static constexpr size_t c   = sizeof(V) * 2 * kBatchTailTexels;               // tail: ping-pong
   static constexpr size_t ab  = a > b ? a : b;
   static constexpr size_t kBytes = ab > c ? ab : c;
@@ -94,7 +94,7 @@ __device__ __forceinline__ void runItem(BatchShared<F>& sh, const ImageDesc* __r
 }
 
 template <class F>
-__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(7, 8))) batchRoundKernel(NVPRO_BATCH_PARAMS)
+__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(5, 8))) batchRoundKernel(NVPRO_BATCH_PARAMS)
 {
   __shared__ BatchShared<F> sh;
   F::template fillDual<kGroupBS>(sh.tab);
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(7
 }
 
 template <class F>
-__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(7, 8))) batchQueueKernel(NVPRO_BATCH_PARAMS)
+__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(5, 8))) batchQueueKernel(NVPRO_BATCH_PARAMS)
 {
   __shared__ BatchShared<F> sh;
   __shared__ uint32_t sNext, sLast;

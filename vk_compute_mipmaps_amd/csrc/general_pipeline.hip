@@ -9,7 +9,7 @@ template <class F>
 struct GeneralShared
 {
   typename F::Tables tab;
-  typename F::Val mid[kGeneralMidEdge * kGeneralMidEdge];
+  typename F::Val mid[gen2MidTexels<F>()];
 };
 
 template <class F>
@@ -48,11 +48,11 @@ static hipError_t launchGeneralTyped(hipStream_t stream, const ImageDesc& img, u
   else
   {
     const LevelDesc& d    = img.lv[inLevel + 2u];
-    const uint32_t tilesX = (d.width + 7u) / 8u, tilesY = (d.height + 7u) / 8u;
-    const uint64_t want   = (uint64_t)tilesX * tilesY;
+    uint32_t tilesX       = 0;
+    const uint64_t want   = generalGroupCount(d.width, d.height, 2u, F::kBytes, &tilesX);
     const uint32_t blocks = (uint32_t)(want < cap ? want : cap);
     hipLaunchKernelGGL((generalTwoLevelKernel<F>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
-                       inLevel, tilesX, tilesX * tilesY);
+                       inLevel, tilesX, (uint32_t)want);
   }
   return hipGetLastError();
 }

@@ -156,18 +156,37 @@ __device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2
 // the 2x2 / 2x3 / 3x2 / 3x3 energy-conserving kernel.
 // ---------------------------------------------------------------------------
 
+// Geometry of a two-level general group: a TX x TY tile of the second level,
+// whose first level (up to (2 TX + 1) x (2 TY + 1) texels, sharedLevel_ of
+// nvpro_pyramid.glsl:555) is computed into LDS by threads that each own a
+// vertical run of R first-level texels of one column.  Narrow texels take the
+// wide tile; 16-byte texels a smaller one to bound registers.
+template <uint32_t BYTES>
+struct Gen2Geom
+{
+  static constexpr uint32_t TX = BYTES <= 8u ? 16u : 8u;
+  static constexpr uint32_t TY = 8u;
+  static constexpr uint32_t R  = BYTES <= 8u ? 3u : 2u;
+  static constexpr uint32_t MW = 2u * TX + 1u;  // LDS row stride of the first level
+  static constexpr uint32_t MH = 2u * TY + 1u;
+  static_assert(MW * ((MH + R - 1u) / R) <= (uint32_t)kGroupBS, "one pass of the workgroup");
+};
+
 // Groups of a one-level general step (a thread per output) and of a two-level
-// one (an 8x8 tile of the second level each; *tilesX receives the tile columns).
+// one (a Gen2Geom tile of the second level each; *tilesX receives the tile
+// columns).
 __host__ __device__ inline uint32_t generalGroupCount(uint32_t dstWidth, uint32_t dstHeight, uint32_t count,
-                                                      uint32_t* tilesX)
+                                                      uint32_t bytesPerTexel, uint32_t* tilesX)
 {
   if(count == 1u)
   {
     *tilesX = 0;
     return (uint32_t)(((uint64_t)dstWidth * dstHeight + kGroupBS - 1) / kGroupBS);
   }
-  *tilesX = (dstWidth + 7u) / 8u;
-  return *tilesX * ((dstHeight + 7u) / 8u);
+  const uint32_t tx = bytesPerTexel <= 8u ? Gen2Geom<8>::TX : Gen2Geom<16>::TX;
+  const uint32_t ty = bytesPerTexel <= 8u ? Gen2Geom<8>::TY : Gen2Geom<16>::TY;
+  *tilesX           = (dstWidth + tx - 1u) / tx;
+  return *tilesX * ((dstHeight + ty - 1u) / ty);
 }
 
 // One level (glsl:828-842): outputs [256 g, 256 g + 256) of level inLevel+1, in
@@ -190,52 +209,96 @@ __device__ __forceinline__ void generalOneGroup(const TT& tab, const ImageDesc& 
   storeTexel<F>(texelPtr(img, inLevel + 1u, dx, dy, F::kBytes), F::store(tab, v));
 }
 
-// LDS values of a two-level general group: sharedLevel_ (nvpro_pyramid.glsl:555).
-constexpr uint32_t kGeneralMidEdge = 17;
+// LDS values of a two-level general group.
+template <class F>
+constexpr uint32_t gen2MidTexels()
+{
+  return Gen2Geom<F::kBytes>::MW * Gen2Geom<F::kBytes>::MH;
+}
 
-// Two levels: the 8x8 tile `tile` of level inLevel+2 and the first-level texels
-// it needs.  The first level's (up to) 17x17 tile is computed into `mid`
-// ([y][x], kGeneralMidEdge wide) and written out as well (fillIntermediateTile_,
-// glsl:732-781); then the 8x8 tile from LDS (fillLastTile_, glsl:791-820).
-// Neighbouring tiles recompute their shared first-level border texels and store
-// identical values.  Contains barriers.
+// Two levels: the Gen2Geom tile `tile` of level inLevel+2 and the first-level
+// texels it needs.  First level (fillIntermediateTile_, glsl:732-781): thread t
+// owns column t % MW of the first-level tile, rows R (t / MW) .. + R-1; it
+// issues the loads of its whole input window (up to 2R+1 rows x 3 columns) at
+// once, then decodes each input texel once and computes its run top to bottom
+// (reduceStoreSample_'s order: vertical per column, then horizontal).  Each
+// value is stored and kept in `mid` ([y][x], stride MW).  Second level
+// (fillLastTile_, glsl:791-820) from LDS, a thread per texel.  Neighbouring
+// tiles recompute their shared first-level border texels and store identical
+// values.  Contains barriers.
 template <class F, class TT>
 __device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* mid, const ImageDesc& img,
                                                 uint32_t inLevel, uint32_t tilesX, uint32_t tile)
 {
-  constexpr uint32_t E = kGeneralMidEdge;
-  const LevelDesc& src = img.lv[inLevel];
-  const LevelDesc& l1  = img.lv[inLevel + 1u];
-  const LevelDesc& l2  = img.lv[inLevel + 2u];
+  using G   = Gen2Geom<F::kBytes>;
+  using V   = typename F::Val;
+  using Raw = typename F::Raw;
+  constexpr uint32_t R   = G::R;
+  constexpr uint32_t NIN = 2u * R + 1u;  // input rows of a run of R outputs (3-tall kernel)
+  const LevelDesc& src   = img.lv[inLevel];
+  const LevelDesc& l1    = img.lv[inLevel + 1u];
+  const LevelDesc& l2    = img.lv[inLevel + 2u];
   const int kx1 = kernelSize(src.width), ky1 = kernelSize(src.height);
   const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
-  const uint32_t nx = kx2 == 3 ? 17u : 16u, ny = ky2 == 3 ? 17u : 16u;
+  const uint32_t nx = 2u * G::TX + (kx2 == 3 ? 1u : 0u), ny = 2u * G::TY + (ky2 == 3 ? 1u : 0u);
   const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
-  for(uint32_t e = threadIdx.x; e < nx * ny; e += kGroupBS)
+  const uint32_t lx = threadIdx.x % G::MW, ly0 = (threadIdx.x / G::MW) * R;
+  const uint32_t gx = 2u * G::TX * tx + lx, gy0 = 2u * G::TY * ty + ly0;
+  if(lx < nx && ly0 < ny && gx < l1.width && gy0 < l1.height)
   {
-    const uint32_t ly = e / nx, lx = e - ly * nx;
-    const uint32_t gx = 16u * tx + lx, gy = 16u * ty + ly;
-    if(gx < l1.width && gy < l1.height)
+    uint32_t n = R;  // outputs of this run inside the tile and the level
+    n = ny - ly0 < n ? ny - ly0 : n;
+    n = l1.height - gy0 < n ? l1.height - gy0 : n;
+    const uint32_t rows = 2u * (n - 1u) + (uint32_t)ky1;  // input rows the run reads
+    const uint8_t* p    = img.base + src.offset + (uint64_t)(2u * gy0) * src.pitch + (uint64_t)(2u * gx) * F::kBytes;
+    Raw raw[NIN][3];
+#pragma unroll
+    for(uint32_t r = 0; r < NIN; ++r)
+#pragma unroll
+      for(int j = 0; j < 3; ++j)
+        if(r < rows && j < kx1)
+          raw[r][j] = loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)j * F::kBytes);
+    V top[3];  // decoded input row 2k of output k (the last row of output k-1 when 3-tall)
+#pragma unroll
+    for(int j = 0; j < 3; ++j)
+      if(j < kx1)
+        top[j] = F::template load<false>(tab, raw[0][j]);
+#pragma unroll
+    for(uint32_t k = 0; k < R; ++k)
     {
-      const uint8_t* p = img.base + src.offset + (uint64_t)(2u * gy) * src.pitch + (uint64_t)(2u * gx) * F::kBytes;
-      auto at          = [&](int j, int r) {
-        return F::template load<false>(tab, loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)j * F::kBytes));
-      };
-      const typename F::Val v = generalSample<F>(at, kx1, ky1, l1.width, l1.height, gx, gy);
-      storeTexel<F>(texelPtr(img, inLevel + 1u, gx, gy, F::kBytes), F::store(tab, v));
-      mid[ly * E + lx] = v;
+      if(k < n)
+      {
+        const Weights3 wy = weights3(l1.height, gy0 + k);
+        V h[3];
+#pragma unroll
+        for(int j = 0; j < 3; ++j)
+        {
+          if(j < kx1)
+          {
+            const V a0 = top[j];
+            const V a1 = ky1 >= 2 ? F::template load<false>(tab, raw[2u * k + 1u][j]) : a0;
+            const V a2 = ky1 == 3 ? F::template load<false>(tab, raw[2u * k + 2u][j]) : a0;
+            h[j]       = verticalTap<F>(wy, ky1, a0, a1, a2);
+            if(k + 1u < n)  // the next output's first row: shared (3-tall) or the row after (2-tall)
+              top[j] = ky1 == 3 ? a2 : F::template load<false>(tab, raw[2u * k + 2u][j]);
+          }
+        }
+        const V v = horizontalTap<F>(l1.width, gx, kx1, h[0], h[1], h[2]);
+        storeTexel<F>(texelPtr(img, inLevel + 1u, gx, gy0 + k, F::kBytes), F::store(tab, v));
+        mid[(ly0 + k) * G::MW + lx] = v;
+      }
     }
   }
   __syncthreads();
-  if(threadIdx.x < 64u)
+  if(threadIdx.x < G::TX * G::TY)
   {
-    const uint32_t ox = threadIdx.x & 7u, oy = threadIdx.x >> 3;
-    const uint32_t gx = 8u * tx + ox, gy = 8u * ty + oy;
-    if(gx < l2.width && gy < l2.height)
+    const uint32_t ox = threadIdx.x % G::TX, oy = threadIdx.x / G::TX;
+    const uint32_t x = G::TX * tx + ox, y = G::TY * ty + oy;
+    if(x < l2.width && y < l2.height)
     {
-      auto at                 = [&](int j, int r) { return mid[(2u * oy + r) * E + 2u * ox + j]; };
-      const typename F::Val v = generalSample<F>(at, kx2, ky2, l2.width, l2.height, gx, gy);
-      storeTexel<F>(texelPtr(img, inLevel + 2u, gx, gy, F::kBytes), F::store(tab, v));
+      auto at = [&](int j, int r) { return mid[(2u * oy + r) * G::MW + 2u * ox + j]; };
+      const V v = generalSample<F>(at, kx2, ky2, l2.width, l2.height, x, y);
+      storeTexel<F>(texelPtr(img, inLevel + 2u, x, y, F::kBytes), F::store(tab, v));
     }
   }
   __syncthreads();

@@ -133,9 +133,38 @@ __device__ __forceinline__ Weights3 weights3(uint32_t n, uint32_t d)
   return Weights3{w0, w1, (1.0f - w0) - w1};
 }
 
-// reduceStoreSample_ (nvpro_pyramid.glsl:575-656) without the store: the sample
-// of destination texel (dx, dy) of a (dw x dh) level from a kx x ky window whose
-// texel at (column j, row i) `at(j, i)` returns.  Vertical first, then horizontal.
+// The two halves of reduceStoreSample_ (nvpro_pyramid.glsl:575-656): the
+// vertical reduction of one column of a k-tall window (REDUCE of 3 with the
+// destination row's weights, REDUCE2, or a copy), then the horizontal one of the
+// column results.
+template <class F>
+__device__ __forceinline__ typename F::Val verticalTap(const Weights3& wy, int ky, typename F::Val a0,
+                                                       typename F::Val a1, typename F::Val a2)
+{
+  if(ky == 3)
+    return F::reduce3(wy.w0, a0, wy.w1, a1, wy.w2, a2);
+  if(ky == 2)
+    return reduce2<F>(a0, a1);
+  return a0;
+}
+
+template <class F>
+__device__ __forceinline__ typename F::Val horizontalTap(uint32_t dw, uint32_t dx, int kx, typename F::Val h0,
+                                                         typename F::Val h1, typename F::Val h2)
+{
+  if(kx == 3)
+  {
+    const Weights3 wx = weights3(dw, dx);
+    return F::reduce3(wx.w0, h0, wx.w1, h1, wx.w2, h2);
+  }
+  if(kx == 2)
+    return reduce2<F>(h0, h1);
+  return h0;
+}
+
+// reduceStoreSample_ without the store: the sample of destination texel (dx, dy)
+// of a (dw x dh) level from a kx x ky window whose texel at (column j, row i)
+// `at(j, i)` returns.  Vertical first, then horizontal.
 template <class F, class At>
 __device__ __forceinline__ typename F::Val generalSample(At at, int kx, int ky, uint32_t dw, uint32_t dh, uint32_t dx,
                                                          uint32_t dy)
@@ -148,22 +177,13 @@ __device__ __forceinline__ typename F::Val generalSample(At at, int kx, int ky, 
   {
     if(j < kx)
     {
-      if(ky == 3)
-        h[j] = F::reduce3(wy.w0, at(j, 0), wy.w1, at(j, 1), wy.w2, at(j, 2));
-      else if(ky == 2)
-        h[j] = reduce2<F>(at(j, 0), at(j, 1));
-      else
-        h[j] = at(j, 0);
+      const V a0 = at(j, 0);
+      const V a1 = ky >= 2 ? at(j, 1) : a0;
+      const V a2 = ky == 3 ? at(j, 2) : a0;
+      h[j]       = verticalTap<F>(wy, ky, a0, a1, a2);
     }
   }
-  if(kx == 3)
-  {
-    const Weights3 wx = weights3(dw, dx);
-    return F::reduce3(wx.w0, h[0], wx.w1, h[1], wx.w2, h[2]);
-  }
-  if(kx == 2)
-    return reduce2<F>(h[0], h[1]);
-  return h[0];
+  return horizontalTap<F>(dw, dx, kx, h[0], h[1], h[2]);
 }
 
 // Cross-lane moves of any value made of 32-bit words.

@@ -36,7 +36,7 @@ uint64_t groupInputBytes(uint32_t kind, uint32_t bpt)
   if(kind == kSegGeneral1)
     return (uint64_t)kGroupBS * 4u * bpt;
   if(kind == kSegGeneral2)
-    return 34ull * 34ull * bpt;
+    return bpt <= 8u ? 67ull * 35ull * bpt : 35ull * 35ull * bpt;
   return 1u;
 }
 
@@ -119,7 +119,7 @@ int nvpro_pyramid::buildBatchPlan(const NvproPyramidPipelines* pipelines, const 
       {
         const LevelDesc& out = d.lv[st.inputLevel + st.levelCount];
         q.kind               = st.levelCount == 1u ? kSegGeneral1 : kSegGeneral2;
-        groups               = generalGroupCount(out.width, out.height, st.levelCount, &q.tilesX);
+        groups               = generalGroupCount(out.width, out.height, st.levelCount, bpt, &q.tilesX);
         q.numTiles           = groups;
       }
       perImage[i].push_back(Pending{q, groups, s});
