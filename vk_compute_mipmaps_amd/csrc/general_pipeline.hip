@@ -33,14 +33,28 @@ __global__ void __launch_bounds__(kGroupBS) generalTwoLevelKernel(ImageDesc img,
     generalTwoGroup<F>(sh.tab, sh.mid, img, inLevel, tilesX, tile);
 }
 
+// Workgroups of `kernel` that are resident on one compute unit at once.  A
+// persistent launch is capped there: more workgroups would only start after
+// the first ones had finished all their tiles.
+template <class K>
+static uint32_t residentPerCU(K kernel)
+{
+  int n = 0;
+  if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kGroupBS, 0) != hipSuccess || n <= 0)
+    n = 1;
+  return (uint32_t)n;
+}
+
 template <class F>
 static hipError_t launchGeneralTyped(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t count)
 {
-  const uint64_t cap = (uint64_t)deviceComputeUnits() * 8u;
+  static const uint32_t oneFit = residentPerCU(generalOneLevelKernel<F>);
+  static const uint32_t twoFit = residentPerCU(generalTwoLevelKernel<F>);
   if(count == 1u)
   {
     const LevelDesc& d    = img.lv[inLevel + 1u];
     const uint64_t want   = ((uint64_t)d.width * d.height + kGroupBS - 1) / kGroupBS;
+    const uint64_t cap    = (uint64_t)deviceComputeUnits() * oneFit;
     const uint32_t blocks = (uint32_t)(want < cap ? want : cap);
     hipLaunchKernelGGL((generalOneLevelKernel<F>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
                        inLevel, (uint32_t)want);
@@ -50,6 +64,7 @@ static hipError_t launchGeneralTyped(hipStream_t stream, const ImageDesc& img, u
     const LevelDesc& d    = img.lv[inLevel + 2u];
     uint32_t tilesX       = 0;
     const uint64_t want   = generalGroupCount(d.width, d.height, 2u, F::kBytes, &tilesX);
+    const uint64_t cap    = (uint64_t)deviceComputeUnits() * twoFit;
     const uint32_t blocks = (uint32_t)(want < cap ? want : cap);
     hipLaunchKernelGGL((generalTwoLevelKernel<F>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
                        inLevel, tilesX, (uint32_t)want);

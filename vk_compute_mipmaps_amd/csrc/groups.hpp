@@ -48,13 +48,43 @@ __host__ __device__ inline uint32_t fastGroupCount(uint32_t width, uint32_t heig
   return (tiles + perWG - 1u) / perWG;
 }
 
+// The input block of one lane for group g of a fast step: its 2^LB rows, all
+// loads issued before any arithmetic.
+template <class F, int COUNT>
+struct FastWindow
+{
+  typename F::Raw raw[1 << FastGeom<COUNT>::LB][1 << FastGeom<COUNT>::LB];
+};
+
+template <class F, int COUNT>
+__device__ __forceinline__ void fastLoad(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t numTiles,
+                                         uint32_t g, FastWindow<F, COUNT>& w)
+{
+  using G           = FastGeom<COUNT>;
+  constexpr int LB  = G::LB;
+  constexpr int BLK = 1 << LB;
+  const uint32_t tile = g * G::TILES_PER_WG + (threadIdx.x >> G::TEAM);
+  if(tile >= numTiles)
+    return;
+  const uint32_t inTile = threadIdx.x & ((1u << G::TEAM) - 1u);
+  const uint32_t px = compactEven(inTile), py = compactEven(inTile >> 1);
+  const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
+  const uint32_t x0 = (tx << COUNT) + (px << LB), y0 = (ty << COUNT) + (py << LB);
+  const LevelDesc& in = img.lv[inLevel];
+  const uint8_t* p    = img.base + in.offset + (uint64_t)y0 * in.pitch + (uint64_t)x0 * F::kBytes;
+#pragma unroll
+  for(int r = 0; r < BLK; ++r)
+    loadRow<F, BLK>(p + (uint64_t)r * in.pitch, w.raw[r]);
+}
+
 // Group g of a fast step of `img` from level inLevel (tilesX x tilesY tiles in
-// all, numTiles = their product).  `level2`: kGroupBS values of LDS scratch
-// (unused when COUNT < 3).  Contains barriers when COUNT >= 3.
+// all, numTiles = their product), from the lane blocks fastLoad put in `w`.
+// `level2`: kGroupBS values of LDS scratch (unused when COUNT < 3).  Contains
+// barriers when COUNT >= 3.
 template <class F, int COUNT, class TT>
-__device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2, const ImageDesc& img,
-                                          uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
-                                          uint32_t iter)
+__device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* level2, const ImageDesc& img,
+                                            uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
+                                            uint32_t iter, const FastWindow<F, COUNT>& w)
 {
   using G            = FastGeom<COUNT>;
   using Raw          = typename F::Raw;
@@ -68,7 +98,6 @@ __device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2
   const uint32_t inTile = threadIdx.x & ((1u << G::TEAM) - 1u);
   const uint32_t px     = compactEven(inTile);
   const uint32_t py     = compactEven(inTile >> 1);
-  const LevelDesc& in   = img.lv[inLevel];
 
   // Levels 1 and 2 of this thread's block.
   const uint32_t tile = g * G::TILES_PER_WG + (threadIdx.x >> G::TEAM);
@@ -77,11 +106,7 @@ __device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2
   {
     const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
     const uint32_t x0 = (tx << COUNT) + (px << LB), y0 = (ty << COUNT) + (py << LB);
-    Raw raw[BLK][BLK];
-    const uint8_t* p = img.base + in.offset + (uint64_t)y0 * in.pitch + (uint64_t)x0 * F::kBytes;
-#pragma unroll
-    for(int r = 0; r < BLK; ++r)
-      loadRow<F, BLK>(p + (uint64_t)r * in.pitch, raw[r]);
+    const auto& raw = w.raw;
 
     // Level 1: LOAD_REDUCE4 as four loads (premultiplied by 0.25 when exact).
     V l1[BLK / 2][BLK / 2];
@@ -151,6 +176,17 @@ __device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2
   }
 }
 
+// Group g, loads included.  Contains barriers when COUNT >= 3.
+template <class F, int COUNT, class TT>
+__device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2, const ImageDesc& img,
+                                          uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
+                                          uint32_t iter)
+{
+  FastWindow<F, COUNT> w;
+  fastLoad<F, COUNT>(img, inLevel, tilesX, numTiles, g, w);
+  fastCompute<F, COUNT>(tab, level2, img, inLevel, tilesX, numTiles, g, iter, w);
+}
+
 // ---------------------------------------------------------------------------
 // General pipeline (nvpro_pyramid.glsl:536-884): 1 or 2 levels of any size with
 // the 2x2 / 2x3 / 3x2 / 3x3 energy-conserving kernel.
@@ -205,7 +241,7 @@ __device__ __forceinline__ void generalOneGroup(const TT& tab, const ImageDesc& 
   auto at           = [&](int j, int r) {
     return F::template load<false>(tab, loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)j * F::kBytes));
   };
-  const typename F::Val v = generalSample<F>(at, kx, ky, dst.width, dst.height, dx, dy);
+  const typename F::Val v = generalSample<F>(at, kx, ky, axis3(dst.width), axis3(dst.height), dx, dy);
   storeTexel<F>(texelPtr(img, inLevel + 1u, dx, dy, F::kBytes), F::store(tab, v));
 }
 
@@ -218,57 +254,94 @@ constexpr uint32_t gen2MidTexels()
 
 // Two levels: the Gen2Geom tile `tile` of level inLevel+2 and the first-level
 // texels it needs.  First level (fillIntermediateTile_, glsl:732-781): thread t
-// owns column t % MW of the first-level tile, rows R (t / MW) .. + R-1; it
-// issues the loads of its whole input window (up to 2R+1 rows x 3 columns) at
-// once, then decodes each input texel once and computes its run top to bottom
-// (reduceStoreSample_'s order: vertical per column, then horizontal).  Each
-// value is stored and kept in `mid` ([y][x], stride MW).  Second level
-// (fillLastTile_, glsl:791-820) from LDS, a thread per texel.  Neighbouring
-// tiles recompute their shared first-level border texels and store identical
-// values.  Contains barriers.
-template <class F, class TT>
-__device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* mid, const ImageDesc& img,
-                                                uint32_t inLevel, uint32_t tilesX, uint32_t tile)
+// owns column t % MW of the first-level tile, rows R (t / MW) .. + R-1.
+// gen2Load issues the loads of the thread's whole input window (up to 2R+1
+// rows x 3 columns) into a Gen2Window; gen2Compute then decodes each input
+// texel once and computes the run top to bottom (reduceStoreSample_'s order:
+// vertical per column, then horizontal), storing each value and keeping it in
+// `mid` ([y][x], stride MW).  Second level (fillLastTile_, glsl:791-820) from
+// LDS, a thread per texel.  Neighbouring tiles recompute their shared
+// first-level border texels and store identical values.  (Keeping a second
+// window in flight across tiles costs two waves of occupancy per SIMD and
+// measured slower.)
+template <class F>
+struct Gen2Window
 {
-  using G   = Gen2Geom<F::kBytes>;
-  using V   = typename F::Val;
-  using Raw = typename F::Raw;
-  constexpr uint32_t R   = G::R;
-  constexpr uint32_t NIN = 2u * R + 1u;  // input rows of a run of R outputs (3-tall kernel)
-  const LevelDesc& src   = img.lv[inLevel];
-  const LevelDesc& l1    = img.lv[inLevel + 1u];
-  const LevelDesc& l2    = img.lv[inLevel + 2u];
-  const int kx1 = kernelSize(src.width), ky1 = kernelSize(src.height);
+  typename F::Raw raw[2u * Gen2Geom<F::kBytes>::R + 1u][3];
+};
+
+struct Gen2Run
+{
+  uint32_t gx, gy0, n;  // first-level column, first row, outputs of the run (0: idle thread)
+};
+
+template <class F>
+__device__ __forceinline__ Gen2Run gen2Run(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile)
+{
+  using G              = Gen2Geom<F::kBytes>;
+  const LevelDesc& l1  = img.lv[inLevel + 1u];
   const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
   const uint32_t nx = 2u * G::TX + (kx2 == 3 ? 1u : 0u), ny = 2u * G::TY + (ky2 == 3 ? 1u : 0u);
   const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
-  const uint32_t lx = threadIdx.x % G::MW, ly0 = (threadIdx.x / G::MW) * R;
-  const uint32_t gx = 2u * G::TX * tx + lx, gy0 = 2u * G::TY * ty + ly0;
-  if(lx < nx && ly0 < ny && gx < l1.width && gy0 < l1.height)
+  const uint32_t lx = threadIdx.x % G::MW, ly0 = (threadIdx.x / G::MW) * G::R;
+  Gen2Run run{2u * G::TX * tx + lx, 2u * G::TY * ty + ly0, 0u};
+  if(lx < nx && ly0 < ny && run.gx < l1.width && run.gy0 < l1.height)
   {
-    uint32_t n = R;  // outputs of this run inside the tile and the level
-    n = ny - ly0 < n ? ny - ly0 : n;
-    n = l1.height - gy0 < n ? l1.height - gy0 : n;
-    const uint32_t rows = 2u * (n - 1u) + (uint32_t)ky1;  // input rows the run reads
-    const uint8_t* p    = img.base + src.offset + (uint64_t)(2u * gy0) * src.pitch + (uint64_t)(2u * gx) * F::kBytes;
-    Raw raw[NIN][3];
+    uint32_t n = G::R;
+    n      = ny - ly0 < n ? ny - ly0 : n;
+    run.n  = l1.height - run.gy0 < n ? l1.height - run.gy0 : n;
+  }
+  return run;
+}
+
+template <class F>
+__device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile,
+                                         Gen2Window<F>& w)
+{
+  constexpr uint32_t NIN = 2u * Gen2Geom<F::kBytes>::R + 1u;
+  const Gen2Run run      = gen2Run<F>(img, inLevel, tilesX, tile);
+  if(run.n == 0u)
+    return;
+  const LevelDesc& src = img.lv[inLevel];
+  const int kx1 = kernelSize(src.width), ky1 = kernelSize(src.height);
+  const uint32_t rows = 2u * (run.n - 1u) + (uint32_t)ky1;  // input rows the run reads
+  const uint8_t* p = img.base + src.offset + (uint64_t)(2u * run.gy0) * src.pitch + (uint64_t)(2u * run.gx) * F::kBytes;
 #pragma unroll
-    for(uint32_t r = 0; r < NIN; ++r)
+  for(uint32_t r = 0; r < NIN; ++r)
 #pragma unroll
-      for(int j = 0; j < 3; ++j)
-        if(r < rows && j < kx1)
-          raw[r][j] = loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)j * F::kBytes);
+    for(int j = 0; j < 3; ++j)
+      if(r < rows && j < kx1)
+        w.raw[r][j] = loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)j * F::kBytes);
+}
+
+// Contains barriers.
+template <class F, class TT>
+__device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid, const ImageDesc& img,
+                                            uint32_t inLevel, uint32_t tilesX, uint32_t tile, const Gen2Window<F>& w)
+{
+  using G              = Gen2Geom<F::kBytes>;
+  using V              = typename F::Val;
+  const LevelDesc& src = img.lv[inLevel];
+  const LevelDesc& l1  = img.lv[inLevel + 1u];
+  const LevelDesc& l2  = img.lv[inLevel + 2u];
+  const int kx1 = kernelSize(src.width), ky1 = kernelSize(src.height);
+  const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
+  const Gen2Run run = gen2Run<F>(img, inLevel, tilesX, tile);
+  if(run.n != 0u)
+  {
+    const Axis3 ax = axis3(l1.width), ay = axis3(l1.height);
+    const uint32_t lx = threadIdx.x % G::MW, ly0 = (threadIdx.x / G::MW) * G::R;
     V top[3];  // decoded input row 2k of output k (the last row of output k-1 when 3-tall)
 #pragma unroll
     for(int j = 0; j < 3; ++j)
       if(j < kx1)
-        top[j] = F::template load<false>(tab, raw[0][j]);
+        top[j] = F::template load<false>(tab, w.raw[0][j]);
 #pragma unroll
-    for(uint32_t k = 0; k < R; ++k)
+    for(uint32_t k = 0; k < G::R; ++k)
     {
-      if(k < n)
+      if(k < run.n)
       {
-        const Weights3 wy = weights3(l1.height, gy0 + k);
+        const Weights3 wy = weights3(ay, run.gy0 + k);
         V h[3];
 #pragma unroll
         for(int j = 0; j < 3; ++j)
@@ -276,15 +349,15 @@ __device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* 
           if(j < kx1)
           {
             const V a0 = top[j];
-            const V a1 = ky1 >= 2 ? F::template load<false>(tab, raw[2u * k + 1u][j]) : a0;
-            const V a2 = ky1 == 3 ? F::template load<false>(tab, raw[2u * k + 2u][j]) : a0;
+            const V a1 = ky1 >= 2 ? F::template load<false>(tab, w.raw[2u * k + 1u][j]) : a0;
+            const V a2 = ky1 == 3 ? F::template load<false>(tab, w.raw[2u * k + 2u][j]) : a0;
             h[j]       = verticalTap<F>(wy, ky1, a0, a1, a2);
-            if(k + 1u < n)  // the next output's first row: shared (3-tall) or the row after (2-tall)
-              top[j] = ky1 == 3 ? a2 : F::template load<false>(tab, raw[2u * k + 2u][j]);
+            if(k + 1u < run.n)  // the next output's first row: shared (3-tall) or the row after (2-tall)
+              top[j] = ky1 == 3 ? a2 : F::template load<false>(tab, w.raw[2u * k + 2u][j]);
           }
         }
-        const V v = horizontalTap<F>(l1.width, gx, kx1, h[0], h[1], h[2]);
-        storeTexel<F>(texelPtr(img, inLevel + 1u, gx, gy0 + k, F::kBytes), F::store(tab, v));
+        const V v = horizontalTap<F>(ax, run.gx, kx1, h[0], h[1], h[2]);
+        storeTexel<F>(texelPtr(img, inLevel + 1u, run.gx, run.gy0 + k, F::kBytes), F::store(tab, v));
         mid[(ly0 + k) * G::MW + lx] = v;
       }
     }
@@ -292,16 +365,27 @@ __device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* 
   __syncthreads();
   if(threadIdx.x < G::TX * G::TY)
   {
+    const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
     const uint32_t ox = threadIdx.x % G::TX, oy = threadIdx.x / G::TX;
     const uint32_t x = G::TX * tx + ox, y = G::TY * ty + oy;
     if(x < l2.width && y < l2.height)
     {
       auto at = [&](int j, int r) { return mid[(2u * oy + r) * G::MW + 2u * ox + j]; };
-      const V v = generalSample<F>(at, kx2, ky2, l2.width, l2.height, x, y);
+      const V v = generalSample<F>(at, kx2, ky2, axis3(l2.width), axis3(l2.height), x, y);
       storeTexel<F>(texelPtr(img, inLevel + 2u, x, y, F::kBytes), F::store(tab, v));
     }
   }
   __syncthreads();
+}
+
+// One tile.  Contains barriers.
+template <class F, class TT>
+__device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* mid, const ImageDesc& img,
+                                                uint32_t inLevel, uint32_t tilesX, uint32_t tile)
+{
+  Gen2Window<F> w;
+  gen2Load<F>(img, inLevel, tilesX, tile, w);
+  gen2Compute<F>(tab, mid, img, inLevel, tilesX, tile, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -336,6 +420,7 @@ __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, ty
       const int kx = kernelSize(sl.width), ky = kernelSize(sl.height);
       const int order     = fastOrder(count, r);
       const uint32_t n    = dl.width * dl.height;
+      const Axis3 ax = axis3(dl.width), ay = axis3(dl.height);
       for(uint32_t i = threadIdx.x; i < n; i += kGroupBS)
       {
         const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
@@ -352,7 +437,7 @@ __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, ty
         if(fast)
           v = box<F, false>(at(0, 0), at(1, 0), at(0, 1), at(1, 1), order);
         else
-          v = generalSample<F>(at, kx, ky, dl.width, dl.height, dx, dy);
+          v = generalSample<F>(at, kx, ky, ax, ay, dx, dy);
         const typename F::Raw q = F::store(tab, v);
         storeTexel<F>(img.base + dl.offset + (uint64_t)dy * dl.pitch + (uint64_t)dx * F::kBytes, q);
         if(r == count)

@@ -118,18 +118,26 @@ __device__ __forceinline__ uint32_t compactEven(uint32_t v)
 // kernelSizeFromInputSize_ (nvpro_pyramid.glsl:557-561).
 __host__ __device__ __forceinline__ int kernelSize(uint32_t dim) { return dim == 1u ? 1 : (int)(2u | (dim & 1u)); }
 
-// Vertical weights of reduceStoreSample_ (nvpro_pyramid.glsl:581-586) for
-// destination coordinate d of a level n texels long.
+// Weights of reduceStoreSample_ (nvpro_pyramid.glsl:581-586) along an axis of a
+// destination level n texels long: rcp = 1/(2n+1) once per level and axis, then
+// for destination coordinate d: w0 = rcp (n - d), w1 = rcp n, w2 = (1 - w0) - w1.
+struct Axis3
+{
+  float nf, rcp;
+};
+__device__ __forceinline__ Axis3 axis3(uint32_t n)
+{
+  const float nf = (float)n;
+  return Axis3{nf, 1.0f / (2.0f * nf + 1.0f)};
+}
 struct Weights3
 {
   float w0, w1, w2;
 };
-__device__ __forceinline__ Weights3 weights3(uint32_t n, uint32_t d)
+__device__ __forceinline__ Weights3 weights3(const Axis3& a, uint32_t d)
 {
-  const float nf  = (float)n;
-  const float rcp = 1.0f / (2.0f * nf + 1.0f);
-  const float w0  = rcp * (nf - (float)d);
-  const float w1  = rcp * nf;
+  const float w0 = a.rcp * (a.nf - (float)d);
+  const float w1 = a.rcp * a.nf;
   return Weights3{w0, w1, (1.0f - w0) - w1};
 }
 
@@ -149,12 +157,12 @@ __device__ __forceinline__ typename F::Val verticalTap(const Weights3& wy, int k
 }
 
 template <class F>
-__device__ __forceinline__ typename F::Val horizontalTap(uint32_t dw, uint32_t dx, int kx, typename F::Val h0,
+__device__ __forceinline__ typename F::Val horizontalTap(const Axis3& ax, uint32_t dx, int kx, typename F::Val h0,
                                                          typename F::Val h1, typename F::Val h2)
 {
   if(kx == 3)
   {
-    const Weights3 wx = weights3(dw, dx);
+    const Weights3 wx = weights3(ax, dx);
     return F::reduce3(wx.w0, h0, wx.w1, h1, wx.w2, h2);
   }
   if(kx == 2)
@@ -163,14 +171,15 @@ __device__ __forceinline__ typename F::Val horizontalTap(uint32_t dw, uint32_t d
 }
 
 // reduceStoreSample_ without the store: the sample of destination texel (dx, dy)
-// of a (dw x dh) level from a kx x ky window whose texel at (column j, row i)
-// `at(j, i)` returns.  Vertical first, then horizontal.
+// of a level with axes ax, ay (axis3 of its width and height) from a kx x ky
+// window whose texel at (column j, row i) `at(j, i)` returns.  Vertical first,
+// then horizontal.
 template <class F, class At>
-__device__ __forceinline__ typename F::Val generalSample(At at, int kx, int ky, uint32_t dw, uint32_t dh, uint32_t dx,
-                                                         uint32_t dy)
+__device__ __forceinline__ typename F::Val generalSample(At at, int kx, int ky, const Axis3& ax, const Axis3& ay,
+                                                         uint32_t dx, uint32_t dy)
 {
   using V           = typename F::Val;
-  const Weights3 wy = weights3(dh, dy);
+  const Weights3 wy = weights3(ay, dy);
   V h[3];
 #pragma unroll
   for(int j = 0; j < 3; ++j)
@@ -183,7 +192,7 @@ __device__ __forceinline__ typename F::Val generalSample(At at, int kx, int ky, 
       h[j]       = verticalTap<F>(wy, ky, a0, a1, a2);
     }
   }
-  return horizontalTap<F>(dw, dx, kx, h[0], h[1], h[2]);
+  return horizontalTap<F>(ax, dx, kx, h[0], h[1], h[2]);
 }
 
 // Cross-lane moves of any value made of 32-bit words.
