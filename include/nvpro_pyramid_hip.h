@@ -180,6 +180,25 @@ const char* nvproPyramidGetLastError(void);
  * in [1e-3, 1e3), uniform alpha), bit-identical to vk_compute_mipmaps_amd/synth.py. */
 int nvproPyramidSynthesize(hipStream_t stream, uint32_t format, const NvproPyramidImage* image, uint64_t seed);
 
+/* Premultiplied alpha on level 0, in place (the staging step of
+ * include/scoped_image.hpp:233-252, "-do-premultiply-alpha" of
+ * minimal_mipmaps.cpp:316-367, moved to the device).  SRGBA8: alpha = a/255,
+ * rgb = srgbFromLinear(linearFromSrgb(c) * alpha) with the exact conversions of
+ * the sRGBA8 functor; RGBA16F: rgb * a in FP32, rounded to binary16. */
+int nvproPyramidPremultiplyAlpha(hipStream_t stream, uint32_t format, const NvproPyramidImage* image);
+
+/* writeMipmapsTga (include/mipmap_storage.hpp:441-479): every level of a packed
+ * sRGBA8 chain in host memory to its own TGA file, level 0 to baseFilename and
+ * level l to baseFilename with "l." inserted before the extension
+ * ("out.tga" -> "out.1.tga").  Uncompressed 32-bit, top-left origin.
+ * levelCount 0: full chain. */
+int nvproPyramidWriteTga(const void* hostChain, uint32_t width, uint32_t height, uint32_t levelCount,
+                         const char* baseFilename);
+
+/* Reads a 24/32-bit true-colour TGA (uncompressed or run-length encoded, either
+ * origin) into tight RGBA8 rows.  rgba NULL: only *width and *height. */
+int nvproPyramidReadTga(const char* filename, void* rgba, uint64_t capacity, uint32_t* width, uint32_t* height);
+
 /* ------------------------------------------------------------------------- */
 /* Batched chains.  No reference counterpart: the reference loops over images  */
 /* and calls nvproCmdPyramidDispatch once per image (demo_app/mipmaps_app.cpp: */

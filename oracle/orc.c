@@ -515,6 +515,36 @@ void orc_pyramid_linear_f64(double* chain, uint32_t w, uint32_t h, uint32_t mipL
 }
 
 /* ------------------------------------------------------------------------ */
+/* Premultiplied alpha while staging (include/scoped_image.hpp:233-252)       */
+/* ------------------------------------------------------------------------ */
+
+static float ref_linear_from_srgb(uint32_t arg);
+static uint32_t ref_srgb_from_linear(float arg);
+
+/* alpha = a * (1/255); rgb = srgbFromLinear(linearFromSrgb(c) * alpha); alpha
+ * byte kept.  exact != 0: the FP32 rounding of the exact decode and the exact
+ * encode (the library's conversions); exact == 0: the reference's FP32 powf
+ * formulas of shaders/srgb.h:18-41. */
+void orc_premultiply_srgba8(uint8_t* texels, uint64_t count, int exact)
+{
+  for(uint64_t i = 0; i < count; ++i)
+  {
+    uint8_t* t        = texels + 4u * i;
+    const float alpha = (float)t[3] * (1.0f / 255.0f);
+    for(int c = 0; c < 3; ++c)
+    {
+      if(exact)
+      {
+        const float v = orc_srgb_decode_f32(t[c]) * alpha;
+        t[c]          = (uint8_t)orc_srgb_encode_exact((double)v);
+      }
+      else
+        t[c] = (uint8_t)ref_srgb_from_linear(ref_linear_from_srgb(t[c]) * alpha);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* The reference CPU generator (include/mipmap_storage.hpp:207-414).          */
 /* ------------------------------------------------------------------------ */
 

@@ -65,6 +65,7 @@ double orc_srgb_decode_f64(uint32_t code);
 float orc_srgb_decode_f32(uint32_t code);
 float orc_alpha_decode_f32(uint32_t code);
 uint32_t orc_srgb_encode_exact(double x);
+void orc_premultiply_srgba8(uint8_t* texels, uint64_t count, int exact);
 uint32_t orc_alpha_encode_f32(float a);
 uint32_t orc_alpha_encode_f64(double a);
 /* Smallest float whose exact encode is >= k, for k in 1..255. */

@@ -75,6 +75,7 @@ def lib():
         L.orc_half_from_double.argtypes = [ctypes.c_double]
         L.orc_pyramid_rgba16f.restype = ctypes.c_int
         L.orc_pyramid_rgba16f.argtypes = [p, u32, u32, u32, p, u32, ctypes.c_int]
+        L.orc_premultiply_srgba8.argtypes = [p, u64, ctypes.c_int]
         L.orc_reference_cpu_srgba8.argtypes = [p, u32, u32]
         L.orc_reference_cpu_srgba8_mt.argtypes = [p, u32, u32, u32]
         L.orc_compare_srgba8.restype = u32
@@ -157,6 +158,13 @@ def pyramid_rgba16f(chain: np.ndarray, w, h, steps, mip_levels=0, packed=False) 
 def pyramid_linear_f64(chain: np.ndarray, w, h, mip_levels=0) -> np.ndarray:
     out = np.ascontiguousarray(chain, dtype=np.float64).copy()
     lib().orc_pyramid_linear_f64(_ptr(out), w, h, mip_levels)
+    return out
+
+
+def premultiply_srgba8(texels: np.ndarray, exact=True) -> np.ndarray:
+    """Premultiplied alpha of sRGBA8 texels (scoped_image.hpp:233-252); returns a copy."""
+    out = np.ascontiguousarray(texels, dtype=np.uint8).copy()
+    lib().orc_premultiply_srgba8(_ptr(out), out.size // 4, 1 if exact else 0)
     return out
 
 

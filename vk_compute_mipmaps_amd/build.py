@@ -28,9 +28,11 @@ SOURCES = [
     "tail_pipeline.hip",
     "batch_pipeline.hip",
     "synth.hip",
+    "convert.hip",
     "nvpro_pyramid_dispatch.cpp",
     "nvpro_pyramid_batch.cpp",
     "nvpro_pyramid_server.cpp",
+    "nvpro_pyramid_io.cpp",
 ]
 
 HIPCC_FLAGS = [

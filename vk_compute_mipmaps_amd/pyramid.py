@@ -45,6 +45,9 @@ EXPORTED_SYMBOLS = (
     "nvproPyramidBytesPerTexel",
     "nvproPyramidGetLastError",
     "nvproPyramidSynthesize",
+    "nvproPyramidPremultiplyAlpha",
+    "nvproPyramidWriteTga",
+    "nvproPyramidReadTga",
     "nvproPyramidBatchDefaultConfig",
     "nvproPyramidBatchCreate",
     "nvproPyramidBatchLaunch",
@@ -189,6 +192,13 @@ def lib():
         L.nvproPyramidFastDispatcherEx.argtypes = [vp, ctypes.POINTER(NvproPyramidState), u32, u32]
         L.nvproPyramidSynthesize.restype = ctypes.c_int
         L.nvproPyramidSynthesize.argtypes = [vp, u32, ctypes.POINTER(NvproPyramidImage), ctypes.c_uint64]
+        L.nvproPyramidPremultiplyAlpha.restype = ctypes.c_int
+        L.nvproPyramidPremultiplyAlpha.argtypes = [vp, u32, ctypes.POINTER(NvproPyramidImage)]
+        L.nvproPyramidWriteTga.restype = ctypes.c_int
+        L.nvproPyramidWriteTga.argtypes = [vp, u32, u32, u32, ctypes.c_char_p]
+        L.nvproPyramidReadTga.restype = ctypes.c_int
+        L.nvproPyramidReadTga.argtypes = [ctypes.c_char_p, vp, ctypes.c_uint64, ctypes.POINTER(u32),
+                                          ctypes.POINTER(u32)]
         cfgp = ctypes.POINTER(NvproPyramidBatchConfig)
         for name, args in (
                 ("nvproPyramidBatchDefaultConfig", [cfgp]),
@@ -326,6 +336,35 @@ def synthesize(stream: int, fmt: int, image: NvproPyramidImage, seed: int):
     _check(lib().nvproPyramidSynthesize(ctypes.c_void_p(stream), fmt, ctypes.byref(image), seed))
 
 
+def premultiply_alpha(stream: int, fmt: int, image: NvproPyramidImage):
+    """nvproPyramidPremultiplyAlpha: premultiply level 0 of `image` in place."""
+    _check(lib().nvproPyramidPremultiplyAlpha(ctypes.c_void_p(stream), fmt, ctypes.byref(image)))
+
+
+def write_tga(chain: np.ndarray, width: int, height: int, base_filename: str, levels: int = 0):
+    """writeMipmapsTga: one TGA file per level of a packed sRGBA8 host chain."""
+    a = np.ascontiguousarray(chain, dtype=np.uint8)
+    _check(lib().nvproPyramidWriteTga(a.ctypes.data, width, height, levels, base_filename.encode()))
+
+
+def read_tga(filename: str) -> np.ndarray:
+    """A 24/32-bit true-colour TGA as a (height, width, 4) uint8 RGBA array."""
+    w, h = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(lib().nvproPyramidReadTga(filename.encode(), None, 0, ctypes.byref(w), ctypes.byref(h)))
+    out = np.empty((h.value, w.value, 4), np.uint8)
+    _check(lib().nvproPyramidReadTga(filename.encode(), out.ctypes.data, out.nbytes, ctypes.byref(w),
+                                     ctypes.byref(h)))
+    return out
+
+
+def tga_level_filename(base: str, level: int) -> str:
+    """The file writeMipmapsTga names level `level` of `base`."""
+    if level == 0:
+        return base
+    dot = base.rfind(".")
+    return base[: dot + 1] + str(level) + base[dot:] if dot >= 0 else base + str(level)
+
+
 POISON_SRGBA8 = 0xFFFF00FF  # magenta, opaque (bytes FF 00 FF FF), as the reference's test clears
 
 
@@ -352,6 +391,10 @@ class DeviceMipChain:
         a = np.ascontiguousarray(chain)
         assert a.nbytes == self.nbytes
         self.rt.memcpy_htod(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+
+    def premultiply(self, stream: int = 0):
+        """Premultiply level 0's alpha in place (device)."""
+        premultiply_alpha(stream, self.fmt, self.image)
 
     def synthesize(self, seed: int, stream: int = 0):
         """Write the seeded synthetic level 0 (synth.py's images) on the device."""
