@@ -14,6 +14,10 @@ the host as a baseline.  Beside it:
   batched   512 mixed-size sRGBA8 textures (seed 8): one batched launch against a
             loop of plain per-texture dispatches, timed by the C++ host
             (lib/nvpro_pyramid_demo batch), chains/s and p50/p99 per batch;
+  serving   a Poisson stream of 20,000 requests at 20,000/s (sizes of the
+            mixed-size workload, 70% sRGBA8 / 30% RGBA16F) through the serving
+            loop with an 80 GB budget (lib/nvpro_pyramid_demo serve): chains/s,
+            p50/p99 latency;
   configs   single chains of the other configurations (1024^2; the odd sizes
             4095x2047, 1920x1080, 2047x4097 and 4095^2; 16384^2 sRGBA8; 8192^2
             RGBA16F with FP32 and packed-FP16 arithmetic): us and GB/s.
@@ -48,6 +52,7 @@ def parse():
                     help="dispatch only the 6-level fast step over level 0 (for profiling)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     return ap.parse_args()
 
 
@@ -145,6 +150,7 @@ def main():
     configs = None if args.no_configs else large_configs(stream)
     hip_rt.stream_destroy(stream)
     batched = None if args.no_batched else batched_leg()
+    serving = None if args.no_serving else serving_leg()
 
     traffic = None  # HBM bytes per F(0,6) launch, from the rocprofv3 --pmc summary
     tpath = os.path.join(REPO, "profiles", "fast_pipeline_srgba8_4096.md")
@@ -199,6 +205,7 @@ def main():
         "copy_GBps": round(copy_gbs, 1),
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
         "configs": configs,
     }
     print(json.dumps(out))
@@ -211,6 +218,21 @@ def batched_leg():
     from vk_compute_mipmaps_amd import build
 
     cmd = [build.DEMO_PATH, "batch", "--count", "512", "--seed", "8", "--reps", "30", "--autotune"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+    if r.returncode != 0:
+        raise RuntimeError(f"{' '.join(cmd)} failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
+    out = json.loads(r.stdout)
+    out["command"] = "vk_compute_mipmaps_amd/lib/nvpro_pyramid_demo " + " ".join(cmd[1:])
+    return out
+
+
+def serving_leg():
+    """Poisson request stream through the serving loop (C++ host)."""
+    import subprocess
+
+    from vk_compute_mipmaps_amd import build
+
+    cmd = [build.DEMO_PATH, "serve", "--rate", "20000", "--count", "20000", "--seed", "9"]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
     if r.returncode != 0:
         raise RuntimeError(f"{' '.join(cmd)} failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")

@@ -286,7 +286,7 @@ typedef struct NvproPyramidServerConfig {
   uint32_t maxBatchImages;      /* images per batch; default 1024 */
   uint32_t arithmetic16f;       /* NVPRO_PYRAMID_ARITH_* of RGBA16F requests; default FP32 */
   uint32_t reserved[2];         /* must be 0 */
-  NvproPyramidBatchConfig batch; /* work-list configuration; useGraph must be 0 */
+  NvproPyramidBatchConfig batch; /* work-list configuration (default: rounds mode); useGraph must be 0 */
 } NvproPyramidServerConfig;
 
 typedef struct NvproPyramidRequest {

@@ -56,4 +56,22 @@ private:
   std::vector<uint32_t> roundEnd_;
 };
 
+// A grow-only device buffer with pinned staging: upload() copies `bytes` of
+// host data to the device on `stream`.  Like BatchRunner::stage, it must not be
+// called again before the previous copy and its users have finished.
+class DeviceStaging
+{
+public:
+  DeviceStaging() = default;
+  DeviceStaging(const DeviceStaging&) = delete;
+  DeviceStaging& operator=(const DeviceStaging&) = delete;
+  ~DeviceStaging();
+  hipError_t upload(const void* host, size_t bytes, hipStream_t stream, void** device);
+
+private:
+  void* dev_ = nullptr;
+  void* pinned_ = nullptr;
+  size_t cap_ = 0;
+};
+
 }  // namespace nvpro_pyramid

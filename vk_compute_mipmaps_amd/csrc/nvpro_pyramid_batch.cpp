@@ -292,6 +292,36 @@ hipError_t BatchRunner::launch(hipStream_t stream) const
   return hipSuccess;
 }
 
+DeviceStaging::~DeviceStaging()
+{
+  if(dev_)
+    (void)hipFree(dev_);
+  if(pinned_)
+    (void)hipHostFree(pinned_);
+}
+
+hipError_t DeviceStaging::upload(const void* host, size_t bytes, hipStream_t stream, void** device)
+{
+  if(bytes > cap_)
+  {
+    if(dev_)
+      (void)hipFree(dev_);
+    if(pinned_)
+      (void)hipHostFree(pinned_);
+    dev_ = pinned_ = nullptr;
+    cap_           = 0;
+    const size_t want = align256(bytes);
+    if(hipError_t e = hipMalloc(&dev_, want))
+      return e;
+    if(hipError_t e = hipHostMalloc(&pinned_, want, hipHostMallocDefault))
+      return e;
+    cap_ = want;
+  }
+  memcpy(pinned_, host, bytes);
+  *device = dev_;
+  return hipMemcpyAsync(dev_, pinned_, bytes, hipMemcpyHostToDevice, stream);
+}
+
 uint32_t* BatchRunner::errorWord() const { return args_.counters ? args_.counters + kCounterError : nullptr; }
 
 struct NvproPyramidBatch

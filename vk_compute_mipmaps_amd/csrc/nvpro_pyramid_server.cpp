@@ -123,6 +123,7 @@ struct Slot
   hipEvent_t done    = nullptr;
   uint32_t* errHost  = nullptr;  // pinned copy of the batch's error flags
   BatchRunner runner;
+  DeviceStaging synth;  // level-0 generator jobs and chunks of the batch
   std::vector<uint64_t> tickets;
   bool busy = false;
 };
@@ -242,23 +243,45 @@ static int launch(NvproPyramidServer* s)
     s->admitted.swap(keep);
     std::vector<NvproPyramidImage> images;
     std::vector<uint32_t> mips;
+    std::vector<SynthJob> jobs;
+    std::vector<SynthChunk> chunks;
+    const uint32_t bpt = nvproPyramidBytesPerTexel(format);
     for(uint64_t t : take)
     {
       Request& r = s->requests[t];
       images.push_back(imageOf(s, r));
       mips.push_back(r.req.mipLevels);
-      int e = 0;
       if(r.req.level0)
       {
-        const uint64_t n = (uint64_t)r.req.width * r.req.height * nvproPyramidBytesPerTexel(format);
+        const uint64_t n = (uint64_t)r.req.width * r.req.height * bpt;
         if(hipError_t he = hipMemcpyAsync(images.back().data, r.req.level0, n, hipMemcpyHostToDevice, slot->stream))
-          e = hostFail(he, "%s: level 0 upload: %s", who, hipGetErrorString(he));
+          return hostFail(he, "%s: level 0 upload: %s", who, hipGetErrorString(he));
       }
       else
-        e = nvproPyramidSynthesize(slot->stream, format, &images.back(), r.req.seed);
-      if(e)
-        return e;
+      {
+        const uint32_t j = (uint32_t)jobs.size();
+        jobs.push_back(SynthJob{(uint8_t*)images.back().data, r.req.seed, r.req.width, r.req.height,
+                                r.req.width * bpt, format});
+        const uint64_t n = (uint64_t)r.req.width * r.req.height;
+        for(uint64_t first = 0; first < n; first += kSynthChunk)
+          chunks.push_back(SynthChunk{j, 0u, first, n - first < kSynthChunk ? n - first : kSynthChunk});
+      }
       r.state = State::Running;
+    }
+    if(!chunks.empty())
+    {
+      // One launch writes every synthetic level 0 of the batch: jobs, then chunks.
+      std::vector<uint8_t> table(jobs.size() * sizeof(SynthJob) + chunks.size() * sizeof(SynthChunk));
+      memcpy(table.data(), jobs.data(), jobs.size() * sizeof(SynthJob));
+      memcpy(table.data() + jobs.size() * sizeof(SynthJob), chunks.data(), chunks.size() * sizeof(SynthChunk));
+      void* dev        = nullptr;
+      hipError_t he    = slot->synth.upload(table.data(), table.size(), slot->stream, &dev);
+      if(he == hipSuccess)
+        he = launchSynthBatch(slot->stream, (const SynthJob*)dev,
+                              (const SynthChunk*)((uint8_t*)dev + jobs.size() * sizeof(SynthJob)),
+                              (uint32_t)chunks.size());
+      if(he != hipSuccess)
+        return hostFail(he, "%s: level 0 generation: %s", who, hipGetErrorString(he));
     }
     const NvproPyramidPipelines pipes{format, 1u,
                                       format == NVPRO_PYRAMID_FORMAT_RGBA16F ? s->cfg.arithmetic16f
@@ -316,7 +339,8 @@ int nvproPyramidServerDefaultConfig(NvproPyramidServerConfig* config)
   config->maxBatchImages = 1024;
   config->arithmetic16f  = NVPRO_PYRAMID_ARITH_FP32;
   defaultBatchConfig(&config->batch);
-  config->batch.useGraph = 0;  // a new work list every batch: direct launches
+  config->batch.mode     = NVPRO_PYRAMID_BATCH_MODE_ROUNDS;  // the faster mode on mixed-size batches
+  config->batch.useGraph = 0;                                // a new work list every batch: direct launches
   return 0;
 }
 

@@ -14,7 +14,7 @@
 //       one hipGraph like a recorded command buffer) against one batched launch,
 //       checks that both write identical bytes, prints one JSON object.
 //   nvpro_pyramid_demo serve [--rate R] [--count N] [--seed S] [--hold-ms H]
-//                            [--budget-gb B] [--in-flight K] [--packed]
+//                            [--budget-gb B] [--in-flight K] [--max-batch M] [--packed]
 //       Serving loop: N requests arrive as a Poisson stream of R per second;
 //       sizes from the mixed-size workload, 70% sRGBA8 / 30% RGBA16F; level 0
 //       is the seeded synthetic image, written on the device at launch.  The
@@ -286,7 +286,7 @@ static int batch(const Args& a)
 struct ServeArgs
 {
   double rate = 20000.0, holdMs = 20.0, budgetGB = 80.0;
-  uint32_t count = 20000, inFlight = 4;
+  uint32_t count = 20000, inFlight = 4, maxBatch = 0;
   uint64_t seed = 9;
   bool packed = false;
 };
@@ -310,6 +310,8 @@ static int serve(const ServeArgs& a)
   CHECK_PYR(nvproPyramidServerDefaultConfig(&cfg));
   cfg.budgetBytes   = (uint64_t)(a.budgetGB * 1e9);
   cfg.maxInFlight   = a.inFlight;
+  if(a.maxBatch)
+    cfg.maxBatchImages = a.maxBatch;
   cfg.arithmetic16f = a.packed ? NVPRO_PYRAMID_ARITH_PACKED_FP16 : NVPRO_PYRAMID_ARITH_FP32;
   NvproPyramidServer* srv = nullptr;
   CHECK_PYR(nvproPyramidServerCreate(&cfg, &srv));
@@ -369,9 +371,10 @@ static int serve(const ServeArgs& a)
          "\"admission_wait_us\": {\"p50\": %.1f, \"p99\": %.1f},\n",
          percentile(latency, 0.5), percentile(latency, 0.9), percentile(latency, 0.99), percentile(latency, 1.0),
          percentile(wait, 0.5), percentile(wait, 0.99));
-  printf(" \"batches\": %llu, \"mean_batch_images\": %.2f, \"peak_resident_bytes\": %llu}\n",
+  printf(" \"batches\": %llu, \"mean_batch_images\": %.2f, \"max_batch_images\": %u, "
+         "\"peak_resident_bytes\": %llu}\n",
          (unsigned long long)st.batches, st.batches ? (double)st.batchedImages / st.batches : 0.0,
-         (unsigned long long)st.peakResidentBytes);
+         cfg.maxBatchImages, (unsigned long long)st.peakResidentBytes);
   return failed == 0 ? 0 : 1;
 }
 
@@ -395,6 +398,8 @@ int main(int argc, char** argv)
         a.budgetGB = atof(argv[++i]);
       else if(o == "--in-flight" && i + 1 < argc)
         a.inFlight = (uint32_t)atoi(argv[++i]);
+      else if(o == "--max-batch" && i + 1 < argc)
+        a.maxBatch = (uint32_t)atoi(argv[++i]);
       else if(o == "--packed")
         a.packed = true;
       else

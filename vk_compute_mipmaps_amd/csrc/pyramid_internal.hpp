@@ -97,6 +97,22 @@ hipError_t launchBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic,
 // Resident workgroups per compute unit of the batched kernel.
 uint32_t batchMaxWorkgroupsPerCU(uint32_t format, uint32_t arithmetic, bool queue);
 
+// Seeded synthetic level 0 (synth.hip): one image per job; a batched launch
+// writes chunks of kSynthChunk texels of many jobs.
+struct SynthJob
+{
+  uint8_t* base;  // level 0
+  uint64_t seed;
+  uint32_t width, height, pitch, format;
+};
+struct SynthChunk
+{
+  uint32_t job, pad;
+  uint64_t first, count;  // texels [first, first + count), row-major
+};
+constexpr uint64_t kSynthChunk = 16384;
+hipError_t launchSynthBatch(hipStream_t stream, const SynthJob* jobs, const SynthChunk* chunks, uint32_t numChunks);
+
 // Launch a validated plan: per-step kernels for the large steps, then one tail
 // launch for the small remainder.
 hipError_t executePlan(hipStream_t stream, const ImageDesc& img, const NvproPyramidStep* steps, uint32_t count);

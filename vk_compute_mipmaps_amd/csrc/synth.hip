@@ -1,8 +1,9 @@
 // Seeded synthetic level 0 on the device: the workload generator of the
 // benchmark and serving inputs, bit-identical to vk_compute_mipmaps_amd/synth.py
 // (same counter-based hash, same integer value noise; RGBA16F radiance through
-// FP64 exp, rounded to half).  Every texel is independent, so one thread per
-// texel, grid-strided.
+// FP64 exp, rounded to half).  Every texel is independent: a workgroup writes
+// chunks of kSynthChunk consecutive texels, one thread per texel; a batched
+// launch covers the chunks of many images.
 #include <hip/hip_fp16.h>
 
 #include "pyramid_internal.hpp"
@@ -23,38 +24,27 @@ __device__ __forceinline__ uint64_t hash4(uint64_t seedMix, uint64_t i, uint64_t
   return mix64(mix64(seedMix ^ i) ^ j);
 }
 
-__global__ void __launch_bounds__(256) synthSrgba8Kernel(uint8_t* base, uint32_t width, uint32_t height,
-                                                         uint32_t pitch, uint64_t seed)
+// Texel (x, y) of the seeded sRGBA8 image; sm[s] = mix64(seed ^ (s << 56)).
+__device__ __forceinline__ uint32_t synthSrgba8Texel(const uint64_t (&sm)[8], uint32_t x, uint32_t y)
 {
-  uint64_t sm[8];
+  const uint64_t i = x >> 6, j = y >> 6, mx = x & 63u, my = y & 63u;
+  const uint64_t D  = 1ull << 18;
+  const uint64_t sx = mx * mx * (192u - 2u * mx), sy = my * my * (192u - 2u * my);
+  uint32_t texel = 0;
 #pragma unroll
-  for(int s = 0; s < 8; ++s)
-    sm[s] = mix64(seed ^ ((uint64_t)s << 56));
-  const uint64_t n      = (uint64_t)width * height;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for(uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+  for(int c = 0; c < 3; ++c)
   {
-    const uint32_t y = (uint32_t)(t / width), x = (uint32_t)(t - (uint64_t)y * width);
-    const uint64_t i = x >> 6, j = y >> 6, mx = x & 63u, my = y & 63u;
-    const uint64_t D  = 1ull << 18;
-    const uint64_t sx = mx * mx * (192u - 2u * mx), sy = my * my * (192u - 2u * my);
-    uint32_t texel = 0;
-#pragma unroll
-    for(int c = 0; c < 3; ++c)
-    {
-      const uint64_t L00 = hash4(sm[c], i, j) >> 56, L10 = hash4(sm[c], i + 1, j) >> 56;
-      const uint64_t L01 = hash4(sm[c], i, j + 1) >> 56, L11 = hash4(sm[c], i + 1, j + 1) >> 56;
-      const uint64_t top = L00 * (D - sx) + L10 * sx, bot = L01 * (D - sx) + L11 * sx;
-      const uint64_t num = top * (D - sy) + bot * sy;
-      const int64_t v    = (int64_t)((num + (1ull << 35)) >> 36);
-      const int64_t nz   = (int64_t)((hash4(sm[4 + c], x, y) >> 32) % 33u) - 16;
-      int64_t q          = v + nz;
-      q                  = q < 0 ? 0 : (q > 255 ? 255 : q);
-      texel |= (uint32_t)q << (8 * c);
-    }
-    texel |= (uint32_t)(hash4(sm[7], x, y) >> 56) << 24;
-    *reinterpret_cast<uint32_t*>(base + (uint64_t)y * pitch + (uint64_t)x * 4u) = texel;
+    const uint64_t L00 = hash4(sm[c], i, j) >> 56, L10 = hash4(sm[c], i + 1, j) >> 56;
+    const uint64_t L01 = hash4(sm[c], i, j + 1) >> 56, L11 = hash4(sm[c], i + 1, j + 1) >> 56;
+    const uint64_t top = L00 * (D - sx) + L10 * sx, bot = L01 * (D - sx) + L11 * sx;
+    const uint64_t num = top * (D - sy) + bot * sy;
+    const int64_t v    = (int64_t)((num + (1ull << 35)) >> 36);
+    const int64_t nz   = (int64_t)((hash4(sm[4 + c], x, y) >> 32) % 33u) - 16;
+    int64_t q          = v + nz;
+    q                  = q < 0 ? 0 : (q > 255 ? 255 : q);
+    texel |= (uint32_t)q << (8 * c);
   }
+  return texel | (uint32_t)(hash4(sm[7], x, y) >> 56) << 24;
 }
 
 // Round-to-nearest-even straight from binary64 (no detour through binary32).
@@ -63,33 +53,137 @@ __device__ __forceinline__ uint16_t f64ToHalfBits(double v)
   return __builtin_bit_cast(uint16_t, (_Float16)v);
 }
 
-__global__ void __launch_bounds__(256) synthRgba16fKernel(uint8_t* base, uint32_t width, uint32_t height,
-                                                          uint32_t pitch, uint64_t seed)
+// Texel (x, y) of the seeded RGBA16F image; sm[0..2] the RGB streams, sm[7] alpha.
+__device__ __forceinline__ uint2 synthRgba16fTexel(const uint64_t (&sm)[8], uint32_t x, uint32_t y)
 {
-  uint64_t sm[4];
+  const double lo    = -6.907755278982137;  // ln 1e-3
+  const double span  = 13.815510557964274;  // ln 1e3 - ln 1e-3
+  const double scale = 0x1p-53;
+  uint16_t h[4];
 #pragma unroll
-  for(int s = 0; s < 3; ++s)
-    sm[s] = mix64(seed ^ ((uint64_t)s << 56));
-  sm[3]                 = mix64(seed ^ (7ull << 56));
-  const double lo       = -6.907755278982137;  // ln 1e-3
-  const double span     = 13.815510557964274;  // ln 1e3 - ln 1e-3
-  const double scale    = 0x1p-53;
-  const uint64_t n      = (uint64_t)width * height;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for(uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+  for(int c = 0; c < 3; ++c)
   {
-    const uint32_t y = (uint32_t)(t / width), x = (uint32_t)(t - (uint64_t)y * width);
-    uint16_t h[4];
-#pragma unroll
-    for(int c = 0; c < 3; ++c)
-    {
-      const double u = (double)(hash4(sm[c], x, y) >> 11) * scale;
-      h[c]           = f64ToHalfBits(exp(lo + u * span));
-    }
-    h[3] = f64ToHalfBits((double)(hash4(sm[3], x, y) >> 11) * scale);
-    uint2 v = make_uint2(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16);
-    *reinterpret_cast<uint2*>(base + (uint64_t)y * pitch + (uint64_t)x * 8u) = v;
+    const double u = (double)(hash4(sm[c], x, y) >> 11) * scale;
+    h[c]           = f64ToHalfBits(exp(lo + u * span));
   }
+  h[3] = f64ToHalfBits((double)(hash4(sm[7], x, y) >> 11) * scale);
+  return make_uint2(h[0] | (uint32_t)h[1] << 16, h[2] | (uint32_t)h[3] << 16);
+}
+
+__device__ __forceinline__ void seedStreams(uint64_t seed, uint64_t (&sm)[8])
+{
+#pragma unroll
+  for(int s = 0; s < 8; ++s)
+    sm[s] = mix64(seed ^ ((uint64_t)s << 56));
+}
+
+// Lattice values of the sRGBA8 value noise a chunk needs, per channel: cell
+// columns 0 .. ni-1 and rows j0 .. j0+nj-1 (one byte each).  Every texel of a
+// 64x64 cell uses the same four corners, so a chunk computes them once.
+constexpr uint32_t kLatticeMax = 4096;  // per channel; a chunk needs at most 3075
+struct LatticeCache
+{
+  uint8_t v[3][kLatticeMax];
+};
+
+// Texel (x, y) of the seeded sRGBA8 image with the lattice from `lat`.
+__device__ __forceinline__ uint32_t synthSrgba8TexelCached(const uint64_t (&sm)[8], const LatticeCache& lat,
+                                                           uint32_t ni, uint32_t j0, uint32_t x, uint32_t y)
+{
+  const uint64_t i = x >> 6, j = (y >> 6) - j0, mx = x & 63u, my = y & 63u;
+  const uint64_t D  = 1ull << 18;
+  const uint64_t sx = mx * mx * (192u - 2u * mx), sy = my * my * (192u - 2u * my);
+  const uint32_t e  = (uint32_t)(j * ni + i);
+  uint32_t texel    = 0;
+#pragma unroll
+  for(int c = 0; c < 3; ++c)
+  {
+    const uint64_t L00 = lat.v[c][e], L10 = lat.v[c][e + 1u], L01 = lat.v[c][e + ni], L11 = lat.v[c][e + ni + 1u];
+    const uint64_t top = L00 * (D - sx) + L10 * sx, bot = L01 * (D - sx) + L11 * sx;
+    const uint64_t num = top * (D - sy) + bot * sy;
+    const int64_t v    = (int64_t)((num + (1ull << 35)) >> 36);
+    const int64_t nz   = (int64_t)((hash4(sm[4 + c], x, y) >> 32) % 33u) - 16;
+    int64_t q          = v + nz;
+    q                  = q < 0 ? 0 : (q > 255 ? 255 : q);
+    texel |= (uint32_t)q << (8 * c);
+  }
+  return texel | (uint32_t)(hash4(sm[7], x, y) >> 56) << 24;
+}
+
+// Texels [first, end) of `job` by the whole workgroup (uniform arguments;
+// contains barriers).
+__device__ __forceinline__ void synthTexels(LatticeCache& lat, const SynthJob& job, uint64_t first, uint64_t end,
+                                            const uint64_t (&sm)[8])
+{
+  const uint32_t w = job.width;
+  if(job.format == NVPRO_PYRAMID_FORMAT_SRGBA8)
+  {
+    const uint32_t y0 = (uint32_t)(first / w), y1 = (uint32_t)((end - 1u) / w);
+    const uint32_t j0 = y0 >> 6, nj = (y1 >> 6) - j0 + 2u, ni = ((w - 1u) >> 6) + 2u;
+    if(ni * nj <= kLatticeMax)
+    {
+      __syncthreads();  // the previous chunk's texels have read the cache
+      for(uint32_t e = threadIdx.x; e < 3u * ni * nj; e += blockDim.x)
+      {
+        const uint32_t c = e / (ni * nj), k = e - c * ni * nj;
+        const uint32_t jj = k / ni, ii = k - jj * ni;
+        lat.v[c][k]       = (uint8_t)(hash4(sm[c], ii, j0 + jj) >> 56);
+      }
+      __syncthreads();
+      for(uint64_t t = first + threadIdx.x; t < end; t += blockDim.x)
+      {
+        const uint32_t y = (uint32_t)(t / w), x = (uint32_t)(t - (uint64_t)y * w);
+        *reinterpret_cast<uint32_t*>(job.base + (uint64_t)y * job.pitch + (uint64_t)x * 4u) =
+            synthSrgba8TexelCached(sm, lat, ni, j0, x, y);
+      }
+      return;
+    }
+  }
+  for(uint64_t t = first + threadIdx.x; t < end; t += blockDim.x)
+  {
+    const uint32_t y = (uint32_t)(t / w), x = (uint32_t)(t - (uint64_t)y * w);
+    uint8_t* p       = job.base + (uint64_t)y * job.pitch;
+    if(job.format == NVPRO_PYRAMID_FORMAT_SRGBA8)
+      *reinterpret_cast<uint32_t*>(p + (uint64_t)x * 4u) = synthSrgba8Texel(sm, x, y);
+    else
+      *reinterpret_cast<uint2*>(p + (uint64_t)x * 8u) = synthRgba16fTexel(sm, x, y);
+  }
+}
+
+// One image: blocks stride over chunks of kSynthChunk texels.
+__global__ void __launch_bounds__(256) synthKernel(SynthJob job)
+{
+  __shared__ LatticeCache lat;
+  uint64_t sm[8];
+  seedStreams(job.seed, sm);
+  const uint64_t n = (uint64_t)job.width * job.height;
+  for(uint64_t first = (uint64_t)blockIdx.x * kSynthChunk; first < n; first += (uint64_t)gridDim.x * kSynthChunk)
+    synthTexels(lat, job, first, first + kSynthChunk < n ? first + kSynthChunk : n, sm);
+}
+
+// Many images: chunks[c] is texels [first, first + count) of jobs[chunks[c].job].
+__global__ void __launch_bounds__(256) synthBatchKernel(const SynthJob* __restrict__ jobs,
+                                                        const SynthChunk* __restrict__ chunks, uint32_t numChunks)
+{
+  __shared__ LatticeCache lat;
+  for(uint32_t c = blockIdx.x; c < numChunks; c += gridDim.x)
+  {
+    const SynthChunk ch = chunks[c];
+    const SynthJob job  = jobs[ch.job];
+    uint64_t sm[8];
+    seedStreams(job.seed, sm);
+    synthTexels(lat, job, ch.first, ch.first + ch.count, sm);
+  }
+}
+
+hipError_t launchSynthBatch(hipStream_t stream, const SynthJob* jobs, const SynthChunk* chunks, uint32_t numChunks)
+{
+  if(numChunks == 0)
+    return hipSuccess;
+  const uint32_t cap  = deviceComputeUnits() * 8u;
+  const uint32_t grid = numChunks < cap ? numChunks : cap;
+  hipLaunchKernelGGL(synthBatchKernel, dim3(grid), dim3(256), 0, stream, jobs, chunks, numChunks);
+  return hipGetLastError();
 }
 
 }  // namespace nvpro_pyramid
@@ -112,17 +206,12 @@ extern "C" int nvproPyramidSynthesize(hipStream_t stream, uint32_t format, const
   if(pitch < (uint64_t)image->width * bpt || pitch > 0xffffffffull || (((uintptr_t)image->data + off) % bpt) != 0 ||
      pitch % bpt != 0)
     return hostFail(hipErrorInvalidValue, "%s: level 0 layout not aligned to %u bytes", who, bpt);
-  uint8_t* base       = (uint8_t*)image->data + off;
+  SynthJob job{(uint8_t*)image->data + off, seed, image->width, image->height, (uint32_t)pitch, format};
   const uint64_t n    = (uint64_t)image->width * image->height;
-  const uint64_t want = (n + 255u) / 256u;
+  const uint64_t want = (n + kSynthChunk - 1u) / kSynthChunk;
   const uint32_t cap  = deviceComputeUnits() * 8u;
   const uint32_t grid = (uint32_t)(want < cap ? want : cap);
-  if(format == NVPRO_PYRAMID_FORMAT_SRGBA8)
-    hipLaunchKernelGGL(synthSrgba8Kernel, dim3(grid), dim3(256), 0, stream, base, image->width, image->height,
-                       (uint32_t)pitch, seed);
-  else
-    hipLaunchKernelGGL(synthRgba16fKernel, dim3(grid), dim3(256), 0, stream, base, image->width, image->height,
-                       (uint32_t)pitch, seed);
+  hipLaunchKernelGGL(synthKernel, dim3(grid), dim3(256), 0, stream, job);
   hipError_t e = hipGetLastError();
   if(e != hipSuccess)
     return hostFail(e, "%s: launch failed: %s", who, hipGetErrorString(e));
