@@ -87,7 +87,9 @@ __device__ __forceinline__ void runItem(BatchShared<F>& sh, const ImageDesc* __r
       case 6: fastGroup<F, 6>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
       case kSegGeneral1: generalOneGroup<F>(sh.tab, img, seg.inputLevel, g); break;
       case kSegGeneral2: generalTwoGroup<F>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, g); break;
-      case kSegTail: tailRun<F>(sh.tab, scratch, scratch + kBatchTailTexels, img, tails[seg.inputLevel]); break;
+      case kSegTail:
+        tailRun<F, kGroupBS>(sh.tab, scratch, scratch + kBatchTailTexels, img, tails[seg.inputLevel]);
+        break;
       default: break;
     }
   }

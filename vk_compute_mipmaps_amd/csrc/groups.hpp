@@ -1,10 +1,11 @@
 // Work-group bodies of the three pipelines, shared by the per-step kernels and
 // the batched kernel.  Each body does one unit of a step ("group"): a tile group
-// of a fast step, 256 outputs or an 8x8 tile of a general step, or a whole tail
-// plan.  They take the LDS decode/encode tables (`tab`, any table type the
+// of a fast step, 256 outputs or a 16x8 (8x8) tile of a general step, or a
+// whole tail plan.  They take the LDS decode/encode tables (`tab`, any table type the
 // functor's load/store accept) and the LDS scratch they need as pointers, so a
 // kernel can overlay the scratch of different bodies.  Every thread of the
-// 256-thread workgroup calls a body; bodies with barriers say so.
+// 256-thread workgroup calls a body (a tail: any workgroup size); bodies with
+// barriers say so.
 #pragma once
 
 #include "kernel_common.hpp"
@@ -399,9 +400,10 @@ __device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* 
 // that of the per-step bodies (fast: box with the reference's order; general:
 // reduceStoreSample_), so results do not depend on which kernel ran a step.
 // `buf0`, `buf1`: LDS ping-pong of maxTexels values each, where maxTexels is at
-// least the first output level's texel count.  Contains barriers.
+// least the first output level's texel count.  BS threads (the workgroup).
+// Contains barriers.
 // ---------------------------------------------------------------------------
-template <class F, class TT>
+template <class F, int BS, class TT>
 __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, typename F::Val* buf1,
                                         const ImageDesc& img, const TailPlan& plan)
 {
@@ -421,7 +423,7 @@ __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, ty
       const int order     = fastOrder(count, r);
       const uint32_t n    = dl.width * dl.height;
       const Axis3 ax = axis3(dl.width), ay = axis3(dl.height);
-      for(uint32_t i = threadIdx.x; i < n; i += kGroupBS)
+      for(uint32_t i = threadIdx.x; i < n; i += BS)
       {
         const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
         auto at           = [&](int j, int k) -> V {

@@ -1,5 +1,6 @@
-// Tail kernel: the small remainder of a plan in one workgroup (tailRun in
-// groups.hpp).
+// Tail kernel: the small remainder of a plan in one workgroup of kTailBS
+// threads (tailRun in groups.hpp): with kTailMaxTexels outputs at most in its
+// first level, every level is one pass of the workgroup.
 #include "groups.hpp"
 
 namespace nvpro_pyramid {
@@ -11,19 +12,21 @@ struct TailShared
   typename F::Val buf[2][MAXT];
 };
 
+constexpr int kTailBS = 1024;
+
 template <class F, int MAXT>
-__global__ void __launch_bounds__(kGroupBS) tailKernel(ImageDesc img, TailPlan plan)
+__global__ void __launch_bounds__(kTailBS) tailKernel(ImageDesc img, TailPlan plan)
 {
   __shared__ TailShared<F, MAXT> sh;
-  F::template fill<kGroupBS>(sh.tab, 1.0f);
+  F::template fill<kTailBS>(sh.tab, 1.0f);
   __syncthreads();
-  tailRun<F>(sh.tab, sh.buf[0], sh.buf[1], img, plan);
+  tailRun<F, kTailBS>(sh.tab, sh.buf[0], sh.buf[1], img, plan);
 }
 
 template <class F>
 static hipError_t launchTailTyped(hipStream_t stream, const ImageDesc& img, const TailPlan& plan)
 {
-  hipLaunchKernelGGL((tailKernel<F, kTailMaxTexels>), dim3(1), dim3(kGroupBS), 0, stream, img, plan);
+  hipLaunchKernelGGL((tailKernel<F, kTailMaxTexels>), dim3(1), dim3(kTailBS), 0, stream, img, plan);
   return hipGetLastError();
 }
 
