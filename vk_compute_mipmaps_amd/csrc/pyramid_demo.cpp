@@ -7,7 +7,7 @@
 //       nvproPyramidDispatch, downloads it and prints each level's size and an
 //       FNV-1a checksum.
 //   nvpro_pyramid_demo batch [--count N] [--seed S] [--format srgba8|rgba16f]
-//                            [--packed] [--reps R] [--autotune]
+//                            [--packed] [--reps R] [--autotune] [--no-graph]
 //       N textures of the mixed-size workload (sides log-uniform on [64, 4096],
 //       40% odd; the same sizes as vk_compute_mipmaps_amd/synth.py mixed_sizes).
 //       Times a loop of plain per-texture dispatches (eager, and captured into
@@ -140,7 +140,7 @@ struct Args
 {
   uint32_t count = 512, reps = 20, format = NVPRO_PYRAMID_FORMAT_SRGBA8, arith = NVPRO_PYRAMID_ARITH_FP32;
   uint64_t seed = 8;
-  bool autotune = false;
+  bool autotune = false, graph = true;
 };
 
 static float elapsedMs(hipEvent_t a, hipEvent_t b)
@@ -208,22 +208,27 @@ static int batch(const Args& a)
     loopMs.push_back(elapsedMs(e0, e1));
   }
   // Loop captured into one graph (every launch recorded ahead, as a command buffer).
-  hipGraph_t g;
-  hipGraphExec_t ge;
-  CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  loopOnce();
-  CHECK_HIP(hipStreamEndCapture(s, &g));
-  CHECK_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-  CHECK_HIP(hipGraphLaunch(ge, s));
+  hipGraph_t g          = nullptr;
+  hipGraphExec_t ge     = nullptr;
   std::vector<double> graphMs;
-  for(uint32_t r = 0; r < a.reps; ++r)
+  if(a.graph)
   {
-    CHECK_HIP(hipEventRecord(e0, s));
+    CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    loopOnce();
+    CHECK_HIP(hipStreamEndCapture(s, &g));
+    CHECK_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     CHECK_HIP(hipGraphLaunch(ge, s));
-    CHECK_HIP(hipEventRecord(e1, s));
-    CHECK_HIP(hipEventSynchronize(e1));
-    graphMs.push_back(elapsedMs(e0, e1));
+    for(uint32_t r = 0; r < a.reps; ++r)
+    {
+      CHECK_HIP(hipEventRecord(e0, s));
+      CHECK_HIP(hipGraphLaunch(ge, s));
+      CHECK_HIP(hipEventRecord(e1, s));
+      CHECK_HIP(hipEventSynchronize(e1));
+      graphMs.push_back(elapsedMs(e0, e1));
+    }
   }
+  else
+    graphMs = loopMs;  // not measured without graphs
 
   // Batched.
   NvproPyramidBatchConfig cfg;
@@ -231,6 +236,7 @@ static int batch(const Args& a)
   float tunedUs = 0;
   if(a.autotune)
     CHECK_PYR(nvproPyramidBatchAutotune(&pipes, images.data(), nullptr, a.count, s, 3, &cfg, &tunedUs));
+  cfg.useGraph = a.graph ? 1u : 0u;
   NvproPyramidBatch* b = nullptr;
   CHECK_PYR(nvproPyramidBatchCreate(&pipes, images.data(), nullptr, a.count, &cfg, &b));
   CHECK_HIP(hipMemsetAsync(pool, 0, padded, s));  // outputs and inputs cleared ...
@@ -276,8 +282,10 @@ static int batch(const Args& a)
   printf(" \"speedup_vs_best_loop\": %.3f, \"identical_to_loop\": %s, \"error_flags\": %u}\n", bestLoop / bMed,
          identical ? "true" : "false", flags);
   CHECK_PYR(nvproPyramidBatchDestroy(b));
-  CHECK_HIP(hipGraphExecDestroy(ge));
-  CHECK_HIP(hipGraphDestroy(g));
+  if(ge)
+    CHECK_HIP(hipGraphExecDestroy(ge));
+  if(g)
+    CHECK_HIP(hipGraphDestroy(g));
   CHECK_HIP(hipFree(pool));
   CHECK_HIP(hipStreamDestroy(s));
   return identical && flags == 0 ? 0 : 1;
@@ -438,6 +446,8 @@ int main(int argc, char** argv)
         a.arith = NVPRO_PYRAMID_ARITH_PACKED_FP16;
       else if(o == "--autotune")
         a.autotune = true;
+      else if(o == "--no-graph")
+        a.graph = false;
       else
       {
         fprintf(stderr, "unknown option %s\n", o.c_str());
