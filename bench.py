@@ -179,7 +179,7 @@ def main():
         "dtype": "u8",
         "data": "synthetic",
         "config": {
-            "workload": "4096x4096 sRGBA8 full chain, 13 levels, plan F(0,6) F(6,6)"
+            "workload": "4096x4096 sRGBA8 full chain, 13 levels, plan F(0,6) F(6,6) (one fused launch)"
             if not args.only_first_step else "4096x4096 sRGBA8, first fast step only F(0,6)",
             "input": f"seeded value noise + uniform noise, seed {SEED}",
             "rotation_buffers": n_buf,
@@ -193,7 +193,7 @@ def main():
         "chain_roofline_frac": round(chain_total_bytes / (us_per_chain * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {
             "bound": "hbm",
-            "kernel": "fastPipelineKernel<Srgba8,6> (F(0,6) over level 0)",
+            "kernel": "fastPipelineKernel<Srgba8, 6, false> (F(0,6) over level 0, launched alone)",
             "achieved": round(k_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -1,5 +1,6 @@
 // Fast pipeline kernels (nvpro_pyramid.glsl:209-534) for gfx950, wave64: one
-// launch per fast step.  The group body (fastGroup) is in groups.hpp.
+// launch per fast step, or per two when the second is a single group or a
+// small tail.  The group body (fastGroup) is in groups.hpp.
 //
 // Workgroups are persistent (the LDS tables are filled once, then the
 // workgroup loops over tile groups) and small enough in registers for eight
@@ -9,67 +10,110 @@
 
 namespace nvpro_pyramid {
 
-template <class F, int COUNT>
+template <class F, int COUNT, bool CONT>
 struct FastShared
 {
   typename F::Tables tab;
-  typename F::Val level2[FastGeom<COUNT>::UPPER ? kGroupBS : 1];
+  typename F::Val level2[(FastGeom<COUNT>::UPPER || CONT) ? kGroupBS : 1];
 };
 
-template <class F, int COUNT>
+// A fused launch (CONT): the step's last level is stored coherently, and the
+// last workgroup to finish (arriveLast) then runs `next`, a fast step of one
+// group with the tables it already holds, or a tail plan with its ping-pong in
+// the level-2 scratch and the tables refilled at the unscaled decode the tail's
+// bodies load with.
+static_assert(2u * kFastTailTexels <= (uint32_t)kGroupBS, "tail ping-pong fits the level-2 scratch");
+
+template <class F, int COUNT, bool CONT>
 __global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(8, 8)))
-fastPipelineKernel(ImageDesc img, uint32_t inLevel, uint32_t tilesX, uint32_t numTiles)
+fastPipelineKernel(ImageDesc img, uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, NextStep next, TailPlan tail)
 {
-  __shared__ FastShared<F, COUNT> sh;
+  __shared__ FastShared<F, COUNT, CONT> sh;
   F::template fill<kGroupBS>(sh.tab, F::kPrescale ? 0.25f : 1.0f);
   __syncthreads();
   const uint32_t numGroups = (numTiles + FastGeom<COUNT>::TILES_PER_WG - 1u) / FastGeom<COUNT>::TILES_PER_WG;
   uint32_t iter            = 0;
   for(uint32_t g = blockIdx.x; g < numGroups; g += gridDim.x, ++iter)
-    fastGroup<F, COUNT>(sh.tab, sh.level2, img, inLevel, tilesX, numTiles, g, iter);
+    fastGroup<F, COUNT, CONT>(sh.tab, sh.level2, img, inLevel, tilesX, numTiles, g, iter);
+  if constexpr(CONT)
+  {
+    if(!arriveLast(next.arrive, gridDim.x, reinterpret_cast<uint32_t*>(sh.level2)))
+      return;
+    if(next.kind == kNextTail)
+    {
+      if(F::kPrescale)
+      {
+        F::template fill<kGroupBS>(sh.tab, 1.0f);
+        __syncthreads();
+      }
+      tailRun<F, kGroupBS>(sh.tab, sh.level2, sh.level2 + kFastTailTexels, img, tail);
+      return;
+    }
+    switch(next.count)
+    {
+      case 1: fastGroup<F, 1>(sh.tab, sh.level2, img, next.inLevel, next.tilesX, next.numTiles, 0u, iter); break;
+      case 2: fastGroup<F, 2>(sh.tab, sh.level2, img, next.inLevel, next.tilesX, next.numTiles, 0u, iter); break;
+      case 3: fastGroup<F, 3>(sh.tab, sh.level2, img, next.inLevel, next.tilesX, next.numTiles, 0u, iter); break;
+      case 4: fastGroup<F, 4>(sh.tab, sh.level2, img, next.inLevel, next.tilesX, next.numTiles, 0u, iter); break;
+      case 5: fastGroup<F, 5>(sh.tab, sh.level2, img, next.inLevel, next.tilesX, next.numTiles, 0u, iter); break;
+      case 6: fastGroup<F, 6>(sh.tab, sh.level2, img, next.inLevel, next.tilesX, next.numTiles, 0u, iter); break;
+      default: break;
+    }
+  }
 }
 
 template <class F, int COUNT>
-static void launchFastCount(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t wgPerCU)
+static void launchFastCount(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t wgPerCU,
+                            const NextStep* next, const TailPlan* tail)
 {
   const LevelDesc& in     = img.lv[inLevel];
   const uint32_t tilesX   = in.width >> COUNT, tilesY = in.height >> COUNT;
   const uint64_t groups   = fastGroupCount(in.width, in.height, COUNT);
   const uint64_t cap      = (uint64_t)deviceComputeUnits() * wgPerCU;
   const uint32_t blocks   = (uint32_t)(groups < cap ? groups : cap);
-  hipLaunchKernelGGL((fastPipelineKernel<F, COUNT>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
-                     inLevel, tilesX, tilesX * tilesY);
+  const TailPlan none{};
+  if(next)
+    hipLaunchKernelGGL((fastPipelineKernel<F, COUNT, true>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream,
+                       img, inLevel, tilesX, tilesX * tilesY, *next, tail ? *tail : none);
+  else
+    hipLaunchKernelGGL((fastPipelineKernel<F, COUNT, false>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream,
+                       img, inLevel, tilesX, tilesX * tilesY, NextStep{}, none);
 }
 
 template <class F>
 static hipError_t launchFastTyped(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t count,
-                                  uint32_t wgPerCU)
+                                  uint32_t wgPerCU, const NextStep* next, const TailPlan* tail)
 {
   switch(count)
   {
-    case 1: launchFastCount<F, 1>(stream, img, inLevel, wgPerCU); break;
-    case 2: launchFastCount<F, 2>(stream, img, inLevel, wgPerCU); break;
-    case 3: launchFastCount<F, 3>(stream, img, inLevel, wgPerCU); break;
-    case 4: launchFastCount<F, 4>(stream, img, inLevel, wgPerCU); break;
-    case 5: launchFastCount<F, 5>(stream, img, inLevel, wgPerCU); break;
-    case 6: launchFastCount<F, 6>(stream, img, inLevel, wgPerCU); break;
+    case 1: launchFastCount<F, 1>(stream, img, inLevel, wgPerCU, next, tail); break;
+    case 2: launchFastCount<F, 2>(stream, img, inLevel, wgPerCU, next, tail); break;
+    case 3: launchFastCount<F, 3>(stream, img, inLevel, wgPerCU, next, tail); break;
+    case 4: launchFastCount<F, 4>(stream, img, inLevel, wgPerCU, next, tail); break;
+    case 5: launchFastCount<F, 5>(stream, img, inLevel, wgPerCU, next, tail); break;
+    case 6: launchFastCount<F, 6>(stream, img, inLevel, wgPerCU, next, tail); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launchFast(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t count, uint32_t wgPerCU)
+hipError_t launchFast(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t count, uint32_t wgPerCU,
+                      const NextStep* next, const TailPlan* tail)
 {
+  if(next && (!next->arrive || (next->kind == kNextFast && (next->count < 1u || next->count > 6u)) ||
+              (next->kind == kNextTail && (!tail || tail->count == 0u || tail->count > NVPRO_PYRAMID_MAX_LEVELS)) ||
+              (next->kind != kNextFast && next->kind != kNextTail)))
+    return hipErrorInvalidValue;
   switch(img.format)
   {
     case NVPRO_PYRAMID_FORMAT_SRGBA8:
-      return launchFastTyped<Srgba8>(stream, img, inLevel, count, wgPerCU);
+      return launchFastTyped<Srgba8>(stream, img, inLevel, count, wgPerCU, next, tail);
     case NVPRO_PYRAMID_FORMAT_RGBA16F:
       return img.arithmetic == NVPRO_PYRAMID_ARITH_PACKED_FP16
-                 ? launchFastTyped<Rgba16fPacked>(stream, img, inLevel, count, wgPerCU)
-                 : launchFastTyped<Rgba16f>(stream, img, inLevel, count, wgPerCU);
+                 ? launchFastTyped<Rgba16fPacked>(stream, img, inLevel, count, wgPerCU, next, tail)
+                 : launchFastTyped<Rgba16f>(stream, img, inLevel, count, wgPerCU, next, tail);
     case NVPRO_PYRAMID_FORMAT_RGBA32F:
-      return launchFastTyped<Rgba32f>(stream, img, inLevel, count, wgPerCU);
+      return launchFastTyped<Rgba32f>(stream, img, inLevel, count, wgPerCU, next, tail);
     default:
       return hipErrorInvalidValue;
   }

@@ -50,25 +50,17 @@ static hipError_t launchGeneralTyped(hipStream_t stream, const ImageDesc& img, u
 {
   static const uint32_t oneFit = residentPerCU(generalOneLevelKernel<F>);
   static const uint32_t twoFit = residentPerCU(generalTwoLevelKernel<F>);
+  const LevelDesc& d           = img.lv[inLevel + count];
+  uint32_t tilesX              = 0;
+  const uint64_t want          = generalGroupCount(d.width, d.height, count, F::kBytes, &tilesX);
+  const uint64_t cap           = (uint64_t)deviceComputeUnits() * (count == 1u ? oneFit : twoFit);
+  const uint32_t blocks        = (uint32_t)(want < cap ? want : cap);
   if(count == 1u)
-  {
-    const LevelDesc& d    = img.lv[inLevel + 1u];
-    const uint64_t want   = ((uint64_t)d.width * d.height + kGroupBS - 1) / kGroupBS;
-    const uint64_t cap    = (uint64_t)deviceComputeUnits() * oneFit;
-    const uint32_t blocks = (uint32_t)(want < cap ? want : cap);
     hipLaunchKernelGGL((generalOneLevelKernel<F>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
                        inLevel, (uint32_t)want);
-  }
   else
-  {
-    const LevelDesc& d    = img.lv[inLevel + 2u];
-    uint32_t tilesX       = 0;
-    const uint64_t want   = generalGroupCount(d.width, d.height, 2u, F::kBytes, &tilesX);
-    const uint64_t cap    = (uint64_t)deviceComputeUnits() * twoFit;
-    const uint32_t blocks = (uint32_t)(want < cap ? want : cap);
     hipLaunchKernelGGL((generalTwoLevelKernel<F>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
                        inLevel, tilesX, (uint32_t)want);
-  }
   return hipGetLastError();
 }
 
@@ -105,23 +97,80 @@ uint32_t deviceComputeUnits()
   return cached[dev];
 }
 
-hipError_t executePlan(hipStream_t stream, const ImageDesc& img, const NvproPyramidStep* steps, uint32_t count)
+static TailPlan tailFrom(const NvproPyramidStep* steps, uint32_t first, uint32_t count)
 {
-  for(uint32_t i = 0; i < count; ++i)
+  TailPlan plan{};
+  for(uint32_t k = first; k < count && plan.count < NVPRO_PYRAMID_MAX_LEVELS; ++k)
+    plan.steps[plan.count++] = TailStep{steps[k].pipeline, steps[k].inputLevel, steps[k].levelCount, 0u};
+  return plan;
+}
+
+static uint64_t firstOutput(const ImageDesc& img, const NvproPyramidStep& s)
+{
+  const LevelDesc& first = img.lv[s.inputLevel + 1u];
+  return (uint64_t)first.width * first.height;
+}
+
+// Whether fusing a successor into fast step s pays, by the step's workgroups
+// (measured on MI355X, sRGBA8): a small grid counts at one arrival counter and
+// gains the successor's launch and start-up (4 workgroups with a one-texel
+// tail: 8.7 -> 7.2 us); a large grid also skips waiting for its own end of
+// kernel, the write-back of everything it stored (2048 workgroups of 4096^2
+// with F(6,6): 28.1 -> 26.5 us per chain); in between, two arrival counts and
+// the continuation cost what they save (256 workgroups of 1024^2 with F(6,4):
+// 8.7 -> 9.2 us).
+static bool fusionPays(const ImageDesc& img, const NvproPyramidStep& s)
+{
+  const LevelDesc& in   = img.lv[s.inputLevel];
+  const uint64_t groups = fastGroupCount(in.width, in.height, s.levelCount);
+  const uint64_t blocks = groups < 8ull * deviceComputeUnits() ? groups : 8ull * deviceComputeUnits();
+  return blocks <= kArriveLeaves || blocks >= 4ull * deviceComputeUnits();
+}
+
+hipError_t executePlan(hipStream_t stream, const ImageDesc& img, const NvproPyramidStep* steps, uint32_t count,
+                       uint32_t* arrive, uint32_t* launches)
+{
+  *launches = 0;
+  for(uint32_t i = 0; i < count; ++i, ++*launches)
   {
     const NvproPyramidStep& s = steps[i];
     // The one-workgroup tail pays off when it saves launches: with a single step
     // left, that step's own kernel is as quick.
-    const LevelDesc& first = img.lv[s.inputLevel + 1u];
-    if((uint64_t)first.width * first.height <= kTailMaxTexels && count - i >= 2u)
+    if(firstOutput(img, s) <= kTailMaxTexels && count - i >= 2u)
     {
-      TailPlan plan{};
-      for(uint32_t k = i; k < count && plan.count < NVPRO_PYRAMID_MAX_LEVELS; ++k)
-        plan.steps[plan.count++] = TailStep{steps[k].pipeline, steps[k].inputLevel, steps[k].levelCount, 0u};
-      return launchTail(stream, img, plan);
+      ++*launches;
+      return launchTail(stream, img, tailFrom(steps, i, count));
     }
-    hipError_t e = s.pipeline == NVPRO_PYRAMID_PIPELINE_FAST ? launchFast(stream, img, s.inputLevel, s.levelCount, 8u)
-                                                             : launchGeneral(stream, img, s.inputLevel, s.levelCount);
+    const bool fast = s.pipeline == NVPRO_PYRAMID_PIPELINE_FAST;
+    if(arrive && i + 1u < count && fast && fusionPays(img, s))
+    {
+      // Fusion: the successor runs in this launch's last workgroup, saving a
+      // launch and the second kernel's start-up (tables, first loads).
+      const NvproPyramidStep& n = steps[i + 1u];
+      const LevelDesc& nin      = img.lv[n.inputLevel];
+      if(n.pipeline == NVPRO_PYRAMID_PIPELINE_FAST && fastGroupCount(nin.width, nin.height, n.levelCount) == 1u)
+      {
+        const uint32_t tilesX = nin.width >> n.levelCount, tilesY = nin.height >> n.levelCount;
+        const NextStep next{arrive, kNextFast, n.inputLevel, n.levelCount, tilesX, tilesX * tilesY};
+        if(hipError_t e = launchFast(stream, img, s.inputLevel, s.levelCount, 8u, &next))
+          return e;
+        ++i;
+        continue;
+      }
+      // A tail small enough for the fast kernel's scratch.  (A general step
+      // followed by a tail measured no faster fused: the tail's first level
+      // then takes several passes of a 256-thread workgroup, where the tail
+      // kernel takes one pass of 1024 threads.)
+      if(firstOutput(img, n) <= kFastTailTexels)
+      {
+        const TailPlan tail = tailFrom(steps, i + 1u, count);
+        const NextStep next{arrive, kNextTail, 0u, 0u, 0u, 0u};
+        ++*launches;
+        return launchFast(stream, img, s.inputLevel, s.levelCount, 8u, &next, &tail);
+      }
+    }
+    hipError_t e = fast ? launchFast(stream, img, s.inputLevel, s.levelCount, 8u)
+                        : launchGeneral(stream, img, s.inputLevel, s.levelCount);
     if(e != hipSuccess)
       return e;
   }

@@ -82,7 +82,7 @@ __device__ __forceinline__ void fastLoad(const ImageDesc& img, uint32_t inLevel,
 // all, numTiles = their product), from the lane blocks fastLoad put in `w`.
 // `level2`: kGroupBS values of LDS scratch (unused when COUNT < 3).  Contains
 // barriers when COUNT >= 3.
-template <class F, int COUNT, class TT>
+template <class F, int COUNT, bool CO = false, class TT>
 __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* level2, const ImageDesc& img,
                                             uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
                                             uint32_t iter, const FastWindow<F, COUNT>& w)
@@ -124,13 +124,16 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
                                F::template load<PRE>(tab, raw[2 * i + 1][2 * j + 1]), fastOrder(COUNT, 1u));
         row[j] = F::store(tab, l1[i][j]);
       }
-      storeRow<F, BLK / 2>(texelPtr(img, inLevel + 1u, x0 >> 1, (y0 >> 1) + i, F::kBytes), row);
+      if constexpr(CO && COUNT == 1)
+        storeTexelCoherent<F>(texelPtr(img, inLevel + 1u, x0 >> 1, (y0 >> 1) + i, F::kBytes), row[0]);
+      else
+        storeRow<F, BLK / 2>(texelPtr(img, inLevel + 1u, x0 >> 1, (y0 >> 1) + i, F::kBytes), row);
     }
     v = l1[0][0];
     if constexpr(LB == 2)
     {
       v = box<F, false>(l1[0][0], l1[0][1], l1[1][0], l1[1][1], fastOrder(COUNT, 2u));
-      storeTexel<F>(texelPtr(img, inLevel + 2u, x0 >> 2, y0 >> 2, F::kBytes), F::store(tab, v));
+      storeTexelTo<F, CO && COUNT == 2>(texelPtr(img, inLevel + 2u, x0 >> 2, y0 >> 2, F::kBytes), F::store(tab, v));
     }
   }
   if constexpr(G::UPPER)
@@ -150,8 +153,9 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
       if(t3 < numTiles)
       {
         const uint32_t ty = t3 / tilesX, tx = t3 - ty * tilesX;
-        storeTexel<F>(texelPtr(img, inLevel + 3u, (tx << (COUNT - 3)) + qx, (ty << (COUNT - 3)) + qy, F::kBytes),
-                      F::store(tab, u));
+        storeTexelTo<F, CO && COUNT == 3>(
+            texelPtr(img, inLevel + 3u, (tx << (COUNT - 3)) + qx, (ty << (COUNT - 3)) + qy, F::kBytes),
+            F::store(tab, u));
 #pragma unroll
         for(int k = 0; k < 3; ++k)
         {
@@ -167,9 +171,12 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
           if((k3 & ((4u << (2 * k)) - 1u)) == 0u)
           {
             const uint32_t sft = (uint32_t)(r - 3);
-            storeTexel<F>(texelPtr(img, inLevel + (uint32_t)r, (tx << (COUNT - r)) + (qx >> sft),
-                                   (ty << (COUNT - r)) + (qy >> sft), F::kBytes),
-                          F::store(tab, u));
+            uint8_t* const p   = texelPtr(img, inLevel + (uint32_t)r, (tx << (COUNT - r)) + (qx >> sft),
+                                          (ty << (COUNT - r)) + (qy >> sft), F::kBytes);
+            if(CO && r == COUNT)
+              storeTexelCoherent<F>(p, F::store(tab, u));
+            else
+              storeTexel<F>(p, F::store(tab, u));
           }
         }
       }
@@ -177,15 +184,17 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
   }
 }
 
-// Group g, loads included.  Contains barriers when COUNT >= 3.
-template <class F, int COUNT, class TT>
+// Group g, loads included.  Contains barriers when COUNT >= 3.  CO: the last
+// level is stored with storeTexelCoherent (for a continuation in the same
+// launch, see arriveLast).
+template <class F, int COUNT, bool CO = false, class TT>
 __device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2, const ImageDesc& img,
                                           uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
                                           uint32_t iter)
 {
   FastWindow<F, COUNT> w;
   fastLoad<F, COUNT>(img, inLevel, tilesX, numTiles, g, w);
-  fastCompute<F, COUNT>(tab, level2, img, inLevel, tilesX, numTiles, g, iter, w);
+  fastCompute<F, COUNT, CO>(tab, level2, img, inLevel, tilesX, numTiles, g, iter, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -403,13 +412,18 @@ __device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* 
 // least the first output level's texel count.  BS threads (the workgroup).
 // Contains barriers.
 // ---------------------------------------------------------------------------
-template <class F, int BS, class TT>
+//
+// The first level reads memory: its outputs go in chunks of PF passes of the
+// workgroup, every load of a chunk issued before its arithmetic, so a chunk
+// pays one memory latency instead of one per pass (9 PF texels of registers).
+template <class F, int BS, int PF = 1, class TT>
 __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, typename F::Val* buf1,
                                         const ImageDesc& img, const TailPlan& plan)
 {
-  using V = typename F::Val;
+  using V      = typename F::Val;
+  using Raw    = typename F::Raw;
   const V* cur = nullptr;  // the current source level in LDS, null: read it from memory
-  V* dst       = buf0;
+  V* dst           = buf0;
   for(uint32_t s = 0; s < plan.count; ++s)
   {
     const uint32_t L     = plan.steps[s].inputLevel;
@@ -423,28 +437,57 @@ __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, ty
       const int order     = fastOrder(count, r);
       const uint32_t n    = dl.width * dl.height;
       const Axis3 ax = axis3(dl.width), ay = axis3(dl.height);
-      for(uint32_t i = threadIdx.x; i < n; i += BS)
-      {
-        const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
-        auto at           = [&](int j, int k) -> V {
-          const uint32_t x = 2u * dx + (uint32_t)j, y = 2u * dy + (uint32_t)k;
-          if(!cur)
-          {
-            const uint8_t* p = img.base + sl.offset + (uint64_t)y * sl.pitch + (uint64_t)x * F::kBytes;
-            return F::template load<false>(tab, loadTexel<F>(p));
-          }
-          return cur[y * sl.width + x];
-        };
-        V v;
-        if(fast)
-          v = box<F, false>(at(0, 0), at(1, 0), at(0, 1), at(1, 1), order);
-        else
-          v = generalSample<F>(at, kx, ky, ax, ay, dx, dy);
-        const typename F::Raw q = F::store(tab, v);
+      auto finish         = [&](uint32_t i, uint32_t dx, uint32_t dy, V v) {
+        const Raw q = F::store(tab, v);
         storeTexel<F>(img.base + dl.offset + (uint64_t)dy * dl.pitch + (uint64_t)dx * F::kBytes, q);
         if(r == count)
           v = F::template load<false>(tab, q);  // the next step reads the stored texel
         dst[i] = v;
+      };
+      if(!cur)
+      {
+        for(uint32_t base = 0; base < n; base += PF * BS)
+        {
+          Raw raw[PF][3][3];
+#pragma unroll
+          for(int p = 0; p < PF; ++p)
+          {
+            const uint32_t i = base + (uint32_t)p * BS + threadIdx.x;
+            if(i < n)
+            {
+              const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
+              const uint8_t* q  = img.base + sl.offset + (uint64_t)(2u * dy) * sl.pitch + (uint64_t)(2u * dx) * F::kBytes;
+#pragma unroll
+              for(int k = 0; k < 3; ++k)
+#pragma unroll
+                for(int j = 0; j < 3; ++j)
+                  if(j < kx && k < ky)
+                    raw[p][k][j] = loadTexel<F>(q + (uint64_t)k * sl.pitch + (uint64_t)j * F::kBytes);
+            }
+          }
+#pragma unroll
+          for(int p = 0; p < PF; ++p)
+          {
+            const uint32_t i = base + (uint32_t)p * BS + threadIdx.x;
+            if(i < n)
+            {
+              const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
+              auto at = [&](int j, int k) -> V { return F::template load<false>(tab, raw[p][k][j]); };
+              finish(i, dx, dy, fast ? box<F, false>(at(0, 0), at(1, 0), at(0, 1), at(1, 1), order)
+                                     : generalSample<F>(at, kx, ky, ax, ay, dx, dy));
+            }
+          }
+        }
+      }
+      else
+      {
+        for(uint32_t i = threadIdx.x; i < n; i += BS)
+        {
+          const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
+          auto at = [&](int j, int k) -> V { return cur[(2u * dy + (uint32_t)k) * sl.width + 2u * dx + (uint32_t)j]; };
+          finish(i, dx, dy, fast ? box<F, false>(at(0, 0), at(1, 0), at(0, 1), at(1, 1), order)
+                                 : generalSample<F>(at, kx, ky, ax, ay, dx, dy));
+        }
       }
       __syncthreads();
       cur = dst;

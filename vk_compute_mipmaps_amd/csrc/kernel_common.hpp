@@ -104,6 +104,83 @@ __device__ __forceinline__ uint8_t* texelPtr(const ImageDesc& img, uint32_t leve
   return img.base + d.offset + (uint64_t)y * d.pitch + (uint64_t)x * bytes;
 }
 
+// A texel store that other workgroups can read once this wave's stores have
+// completed (s_waitcnt): agent-scope relaxed atomic stores of 4 or 8 bytes,
+// which are written through to the device's point of coherence instead of
+// staying dirty in the L2 of one XCD.
+template <class F>
+__device__ __forceinline__ void storeTexelCoherent(uint8_t* p, typename F::Raw v)
+{
+  if constexpr(F::kBytes == 4)
+  {
+    uint32_t w;
+    __builtin_memcpy(&w, &v, 4);
+    __hip_atomic_store(reinterpret_cast<uint32_t*>(p), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  else
+  {
+    uint64_t w[F::kBytes / 8];
+    __builtin_memcpy(w, &v, sizeof w);
+#pragma unroll
+    for(uint32_t i = 0; i < F::kBytes / 8; ++i)
+      __hip_atomic_store(reinterpret_cast<uint64_t*>(p) + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <class F, bool COHERENT>
+__device__ __forceinline__ void storeTexelTo(uint8_t* p, typename F::Raw v)
+{
+  if constexpr(COHERENT)
+    storeTexelCoherent<F>(p, v);
+  else
+    storeTexel<F>(p, v);
+}
+
+// The last workgroup of the grid to get here returns true, the others false.
+// Counters as kArriveLeaves describes.
+// Every thread of every workgroup calls it once, after its last store of the
+// values the last workgroup will read; those stores must be storeTexelCoherent.
+// The last workgroup then sees them: the counts are agent-scope atomics taken
+// after every wave's stores have completed, and the winner drops its caches'
+// lines (agent-scope acquire) before it reads.  No workgroup waits for
+// another, so a grid of any size cannot hang here.  `counters` (kArriveWords)
+// are zero at launch and zero again when the last workgroup returns, ready for
+// the next launch on the same stream.  `flag`: one LDS word.  Contains barriers.
+__device__ __forceinline__ bool arriveLast(uint32_t* counters, uint32_t groups, uint32_t* flag)
+{
+  __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+  if(threadIdx.x == 0)
+  {
+    // Up to kArriveLeaves workgroups count at the root alone (one atomic's
+    // latency on the way to the continuation instead of two).
+    bool last = true;
+    if(groups > kArriveLeaves)
+    {
+      const uint32_t leaf    = blockIdx.x % kArriveLeaves;
+      const uint32_t inLeaf  = (groups - leaf + kArriveLeaves - 1u) / kArriveLeaves;
+      uint32_t* const leafCt = counters + kArriveStride * (1u + leaf);
+      last = __hip_atomic_fetch_add(leafCt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == inLeaf;
+      if(last)
+        __hip_atomic_store(leafCt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if(last)
+    {
+      const uint32_t atRoot = groups > kArriveLeaves ? kArriveLeaves : groups;
+      last = __hip_atomic_fetch_add(counters, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == atRoot;
+      if(last)
+        __hip_atomic_store(counters, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    *flag = last ? 1u : 0u;
+  }
+  __syncthreads();
+  const bool last = *reinterpret_cast<volatile uint32_t*>(flag) != 0u;
+  __syncthreads();  // `flag` may be scratch the caller reuses
+  if(last)
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return last;
+}
+
 // Even bits of v packed together (Morton decode of one coordinate).
 __device__ __forceinline__ uint32_t compactEven(uint32_t v)
 {

@@ -8,7 +8,12 @@
 // vkCmdPipelineBarrier the reference records between dispatches (:180-186).
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "pyramid_internal.hpp"
 
@@ -19,6 +24,7 @@ namespace {
 constexpr uint32_t kMaxSteps = 64;  // far above the <= 15 a 16-level chain can need
 
 thread_local char gLastError[256] = "";
+thread_local uint32_t gLastLaunches = 0;
 
 int vfail(int code, const char* fmt, va_list ap)
 {
@@ -33,6 +39,62 @@ int fail(int code, const char* fmt, ...)
   vfail(code, fmt, ap);
   va_end(ap);
   return code;
+}
+
+// Arrival counters of fused launches (executePlan), one block per stream.  A
+// stream runs its launches one after another, so a block is never in use by
+// two launches at once, and every fused launch leaves it zeroed.  Streams are
+// told apart by hipStreamGetId, which a later stream does not reuse.  There is
+// no block (no fusion) while the stream is being captured: a graph may be
+// launched on any stream, beside eager launches on the capturing one, and
+// would share their block; nor for a stream of another device than the
+// current one, nor beyond kMaxFusedStreams streams, nor when
+// NVPRO_PYRAMID_FUSE=0 is set in the environment.
+constexpr size_t kMaxFusedStreams = 1024;
+
+uint32_t* fusedCounters(hipStream_t stream)
+{
+  static std::mutex lock;
+  static std::map<std::pair<int, unsigned long long>, uint32_t*> blocks;  // by current device, stream id
+  const char* env = getenv("NVPRO_PYRAMID_FUSE");
+  if(env && env[0] == '0')
+    return nullptr;
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  unsigned long long id          = 0;
+  int device                     = 0;
+  if(hipStreamIsCapturing(stream, &capture) != hipSuccess || capture != hipStreamCaptureStatusNone ||
+     hipStreamGetId(stream, &id) != hipSuccess || hipGetDevice(&device) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> guard(lock);
+  const auto key = std::make_pair(device, id);
+  const auto it  = blocks.find(key);
+  if(it != blocks.end())
+    return it->second;
+  int streamDevice = 0;
+  if(blocks.size() >= kMaxFusedStreams || hipStreamGetDevice(stream, &streamDevice) != hipSuccess ||
+     device != streamDevice)
+  {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  void* p            = nullptr;
+  const size_t bytes = sizeof(uint32_t) * kArriveWords;
+  if(hipMalloc(&p, bytes) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if(hipMemsetAsync(p, 0, bytes, stream) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    (void)hipFree(p);
+    return nullptr;
+  }
+  blocks[key] = static_cast<uint32_t*>(p);
+  return static_cast<uint32_t*>(p);
 }
 
 }  // namespace
@@ -410,10 +472,14 @@ extern "C" int nvproPyramidDispatchCustom(hipStream_t stream, const NvproPyramid
     return e;
   if(count != 0)
   {
-    hipError_t e = executePlan(stream, desc, steps, count);
+    uint32_t launches = 0;
+    hipError_t e      = executePlan(stream, desc, steps, count, count >= 2u ? fusedCounters(stream) : nullptr, &launches);
     if(e != hipSuccess)
       return fail(e, "%s: kernel launch failed: %s", who, hipGetErrorString(e));
+    gLastLaunches = launches;
   }
+  else
+    gLastLaunches = 0;
   gLastError[0] = 0;
   return 0;
 }
@@ -424,3 +490,5 @@ extern "C" int nvproPyramidDispatch(hipStream_t stream, const NvproPyramidPipeli
   return nvproPyramidDispatchCustom(stream, pipelines, image, mipLevels, nvproPyramidDefaultGeneralDispatcher,
                                     nvproPyramidDefaultFastDispatcher);
 }
+
+extern "C" uint32_t nvproPyramidLastLaunchCount(void) { return gLastLaunches; }

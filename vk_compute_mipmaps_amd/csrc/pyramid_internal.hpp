@@ -73,9 +73,38 @@ constexpr uint32_t kBatchErrorSpin = 1u;  // a dependency wait gave up
 // ping-pong overlays the other bodies' scratch).
 constexpr uint32_t kBatchTailTexels = 128;
 
-// Per-pipeline launchers.  The caller has validated every step.
+// Arrival counters of a fused launch (arriveLast in kernel_common.hpp): a root
+// and kArriveLeaves leaves, each on its own 256-byte line.  In a grid of more
+// than kArriveLeaves workgroups, workgroup b counts at leaf b % kArriveLeaves
+// and the last workgroup of a leaf counts at the root; a smaller grid counts at
+// the root only.
+// One counter taken by every workgroup of a large grid serialises its atomics
+// at one memory channel (microseconds for a few thousand workgroups); the
+// leaves spread them over many.
+constexpr uint32_t kArriveLeaves = 64;
+constexpr uint32_t kArriveStride = 64;  // words between counters
+constexpr uint32_t kArriveWords  = (1u + kArriveLeaves) * kArriveStride;
+
+// What a fused launch runs in its last workgroup after its own step: a fast
+// step of one group (kNextFast: inLevel, count, tilesX, numTiles) or the tail
+// plan passed beside it (kNextTail).  arrive: kArriveWords zeroed counters that
+// no other launch uses while this one runs.
+constexpr uint32_t kNextFast = 1, kNextTail = 2;
+struct NextStep
+{
+  uint32_t* arrive;
+  uint32_t kind;
+  uint32_t inLevel, count, tilesX, numTiles;
+};
+// Largest first output level of a tail that a fused fast launch runs (its
+// ping-pong overlays the fast step's level-2 scratch).
+constexpr uint32_t kFastTailTexels = 128;
+
+// Per-pipeline launchers.  The caller has validated every step.  With `next`
+// a fast launch also runs what it describes in its last workgroup (`tail`: the
+// plan of a kNextTail).
 hipError_t launchFast(hipStream_t stream, const ImageDesc& img, uint32_t inputLevel, uint32_t levelCount,
-                      uint32_t wavesPerCU);
+                      uint32_t wavesPerCU, const NextStep* next = nullptr, const TailPlan* tail = nullptr);
 hipError_t launchGeneral(hipStream_t stream, const ImageDesc& img, uint32_t inputLevel, uint32_t levelCount);
 hipError_t launchTail(hipStream_t stream, const ImageDesc& img, const TailPlan& plan);
 
@@ -114,8 +143,12 @@ constexpr uint64_t kSynthChunk = 16384;
 hipError_t launchSynthBatch(hipStream_t stream, const SynthJob* jobs, const SynthChunk* chunks, uint32_t numChunks);
 
 // Launch a validated plan: per-step kernels for the large steps, then one tail
-// launch for the small remainder.
-hipError_t executePlan(hipStream_t stream, const ImageDesc& img, const NvproPyramidStep* steps, uint32_t count);
+// launch for the small remainder.  With `arrive` (kArriveWords zeroed counters
+// of the stream), a fast step whose successor is a fast step of one group, or
+// a tail of at most kFastTailTexels first outputs, runs that successor in its
+// own launch.  *launches: the kernel launches made.
+hipError_t executePlan(hipStream_t stream, const ImageDesc& img, const NvproPyramidStep* steps, uint32_t count,
+                       uint32_t* arrive, uint32_t* launches);
 
 // Number of compute units of the current device (cached per device).
 uint32_t deviceComputeUnits();

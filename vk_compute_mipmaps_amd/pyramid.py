@@ -44,6 +44,7 @@ EXPORTED_SYMBOLS = (
     "nvproPyramidPackedLayout",
     "nvproPyramidBytesPerTexel",
     "nvproPyramidGetLastError",
+    "nvproPyramidLastLaunchCount",
     "nvproPyramidSynthesize",
     "nvproPyramidPremultiplyAlpha",
     "nvproPyramidWriteTga",
@@ -171,6 +172,8 @@ def lib():
         L.nvproPyramidBytesPerTexel.argtypes = [u32]
         L.nvproPyramidGetLastError.restype = ctypes.c_char_p
         L.nvproPyramidGetLastError.argtypes = []
+        L.nvproPyramidLastLaunchCount.restype = ctypes.c_uint32
+        L.nvproPyramidLastLaunchCount.argtypes = []
         L.nvproPyramidPackedLayout.restype = ctypes.c_int
         L.nvproPyramidPackedLayout.argtypes = [u32, u32, u32, u32, ctypes.POINTER(ctypes.c_uint64),
                                                ctypes.POINTER(ctypes.c_uint64)]
@@ -230,6 +233,11 @@ def lib():
 
 def last_error() -> str:
     return lib().nvproPyramidGetLastError().decode()
+
+
+def last_launch_count() -> int:
+    """Kernel launches of this thread's last successful dispatch (fused or not)."""
+    return int(lib().nvproPyramidLastLaunchCount())
 
 
 def _check(code: int):
