@@ -53,7 +53,53 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the 512-texture batched workload, in this process, --steps launches (for profiling)")
+    ap.add_argument("--batch-mode", choices=["rounds", "queue"], default="rounds")
+    ap.add_argument("--batch-no-graph", action="store_true", help="launch the batch's kernels directly")
     return ap.parse_args()
+
+
+def batch_only(args):
+    """The batched workload (512 textures, synth.mixed_sizes(512, 8), level 0
+    seeded 8000 + k as in the C++ host) as one nvproPyramidBatch in this
+    process: ms per launch over --steps launches, one JSON line."""
+    from vk_compute_mipmaps_amd import hip_rt, pyramid as P, synth
+
+    hip_rt.check(hip_rt.rt().hipSetDevice(0), "hipSetDevice")
+    stream = hip_rt.stream_create()
+    sizes = synth.mixed_sizes(512, 8)
+    nbytes = [P.chain_bytes(w, h) for w, h in sizes]
+    padded = [(b + 255) // 256 * 256 for b in nbytes]
+    pool = hip_rt.malloc(sum(padded))
+    images, off = [], 0
+    for k, (w, h) in enumerate(sizes):
+        images.append(P.make_image(pool + off, w, h))
+        off += padded[k]
+        P.synthesize(stream, P.FORMAT_SRGBA8, images[-1], 8 * 1000 + k)
+    mode = P.BATCH_MODE_ROUNDS if args.batch_mode == "rounds" else P.BATCH_MODE_QUEUE
+    batch = P.PyramidBatch(P.pipelines(), images,
+                           config=P.batch_config(mode=mode, use_graph=not args.batch_no_graph))
+    for _ in range(max(1, args.warmup)):
+        batch.launch(stream)
+    hip_rt.stream_sync(stream)
+    e0, e1 = hip_rt.event_create(), hip_rt.event_create()
+    hip_rt.event_record(e0, stream)
+    for _ in range(args.steps):
+        batch.launch(stream)
+    hip_rt.event_record(e1, stream)
+    hip_rt.event_sync(e1)
+    ms = hip_rt.event_elapsed_ms(e0, e1) / args.steps
+    info = batch.info()
+    errors = batch.read_errors()
+    batch.destroy()
+    hip_rt.free(pool)
+    print(json.dumps({"workload": "512 mixed sRGBA8 textures, synth.mixed_sizes(512, 8)", "mode": args.batch_mode,
+                      "graph": not args.batch_no_graph, "launches": args.steps, "ms_per_batch": round(ms, 4),
+                      "chains_per_s": round(512 / (ms * 1e-3), 1), "algorithmic_bytes": sum(nbytes),
+                      "GBps": round(sum(nbytes) / (ms * 1e-3) / 1e9, 1), "kernel_launches_per_batch": info.launchCount,
+                      "rounds": info.roundCount, "items": info.itemCount, "workgroups": info.workgroups,
+                      "error_flags": errors}))
 
 
 def main():
@@ -62,6 +108,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path has no multi-GPU leg",
               file=sys.stderr)
         sys.exit(2)
+    if args.batch_only:
+        batch_only(args)
+        return
     import ctypes
 
     import numpy as np

@@ -73,6 +73,20 @@ def test_small_and_ragged_sizes_exact(w, stream):
         _check_exact(w, h, 100 + w * 7 + h, stream)
 
 
+# Widths and heights around the two-level general tile (31 x 7 second-level
+# texels: a first level of 63 x 15, one lane per column of a 64-lane wave) and
+# around thin strips, where a lane lends its column to the lane before it.
+TILE_W = [123, 124, 125, 126, 127, 249, 250, 251, 252, 253, 254, 255, 257, 499, 500, 501, 502]
+TILE_H = [1, 2, 3, 5, 27, 28, 29, 30, 31, 57, 58, 59, 60, 61, 62]
+
+
+@pytest.mark.parametrize("w", TILE_W)
+def test_general_tile_edges_exact(w, stream):
+    for h in TILE_H:
+        _check_exact(w, h, 300 + w + 3 * h, stream)
+        _check_exact(h, w, 301 + w + 3 * h, stream)
+
+
 def test_one_by_one_is_a_no_op(stream):
     """A 1x1 image has a single level: nothing to fill."""
     l0 = H.level0_srgba8(1, 1, 3)

@@ -6,8 +6,9 @@
 // it depends on: step s of an image needs all of step s-1.  A workgroup runs any
 // item kind with the group bodies of groups.hpp: fast steps of 1..6 levels,
 // general steps of 1 or 2 levels, and whole tails.  The bodies' LDS scratch is
-// overlaid; the decode tables are held at both scales (DualTables) so fast and
-// general bodies see exactly the values their per-step kernels see.
+// overlaid; the decode tables (DualTables) serve the fast bodies' premultiplied
+// loads and the general ones' plain loads with exactly the values their
+// per-step kernels see.
 //
 // Queue mode: workgroups are persistent and take the next item from an atomic
 // head counter.  Before an item of step s a workgroup waits for the completion
@@ -96,7 +97,8 @@ __device__ __forceinline__ void runItem(BatchShared<F>& sh, const ImageDesc* __r
 }
 
 template <class F>
-__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(5, 8))) batchRoundKernel(NVPRO_BATCH_PARAMS)
+__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(F::kBytes > 8u ? 4 : 5, 8)))
+batchRoundKernel(NVPRO_BATCH_PARAMS)
 {
   __shared__ BatchShared<F> sh;
   F::template fillDual<kGroupBS>(sh.tab);
@@ -112,7 +114,8 @@ __global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(5
 }
 
 template <class F>
-__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(5, 8))) batchQueueKernel(NVPRO_BATCH_PARAMS)
+__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(F::kBytes > 8u ? 4 : 5, 8)))
+batchQueueKernel(NVPRO_BATCH_PARAMS)
 {
   __shared__ BatchShared<F> sh;
   __shared__ uint32_t sNext, sLast;

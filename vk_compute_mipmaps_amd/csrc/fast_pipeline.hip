@@ -5,7 +5,8 @@
 // Workgroups are persistent (the LDS tables are filled once, then the
 // workgroup loops over tile groups) and small enough in registers for eight
 // waves per SIMD, so the hardware overlaps one wave's loads with another's
-// arithmetic.
+// arithmetic (four for 16-byte texels, whose 4x4 block alone takes 64
+// registers).
 #include "groups.hpp"
 
 namespace nvpro_pyramid {
@@ -25,7 +26,7 @@ struct FastShared
 static_assert(2u * kFastTailTexels <= (uint32_t)kGroupBS, "tail ping-pong fits the level-2 scratch");
 
 template <class F, int COUNT, bool CONT>
-__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(8, 8)))
+__global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(F::kBytes > 8u ? 4 : 8, 8)))
 fastPipelineKernel(ImageDesc img, uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, NextStep next, TailPlan tail)
 {
   __shared__ FastShared<F, COUNT, CONT> sh;

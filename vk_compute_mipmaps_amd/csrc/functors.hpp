@@ -11,7 +11,7 @@
 //   fill<BS>(t, scale)   workgroup-cooperative set-up by BS threads; `scale`
 //                        premultiplies decoded values (0.25 for the 2x2 box)
 //   DualTables,          tables serving load<true> (times 0.25) and
-//   fillDual<BS>(t)      load<false> (times 1) side by side
+//   fillDual<BS>(t)      load<false> (times 1) alike
 //   load<PRE>(t, raw)    NVPRO_PYRAMID_LOAD: Raw -> Val (times 0.25 if PRE)
 //   store(t, v)          NVPRO_PYRAMID_STORE: Val -> Raw
 //   add, mul, reduce3    the arithmetic REDUCE, REDUCE2 and REDUCE4 are built
@@ -173,22 +173,18 @@ struct Srgba8 : Fp32Arith
   }
 
   // Tables for kernels that run premultiplied (fast) and plain (general, tail)
-  // loads alike: dec/adec hold the decode times 0.25, dec1/adec1 times 1.
+  // loads alike: the unscaled tables, with load<true> multiplying by 0.25 after
+  // the lookup.  Scaling by 0.25 is exact for every decoded value (none is
+  // subnormal), so this equals the lookup in a table filled at scale 0.25, bit
+  // for bit, and the LDS holds one set of tables instead of two.
   struct DualTables : Tables
   {
-    float dec1[256];
-    float adec1[256];
   };
 
   template <int BS>
   __device__ static void fillDual(DualTables& t)
   {
-    fill<BS>(t, 0.25f);
-    for(uint32_t i = threadIdx.x; i < 256u; i += BS)
-    {
-      t.dec1[i]  = __uint_as_float(gSrgbDecode[i]);
-      t.adec1[i] = (float)i * (1.0f / 255.0f);
-    }
+    fill<BS>(t, 1.0f);
   }
 
   // texelFetch of the sRGB sampler (preamble:21-22): sRGB decode of RGB and the
@@ -203,9 +199,8 @@ struct Srgba8 : Fp32Arith
   template <bool PRE>
   __device__ static float4 load(const DualTables& t, uint32_t texel)
   {
-    const float* d = PRE ? t.dec : t.dec1;
-    const float* a = PRE ? t.adec : t.adec1;
-    return make_float4(d[texel & 0xffu], d[(texel >> 8) & 0xffu], d[(texel >> 16) & 0xffu], a[texel >> 24]);
+    const float4 v = load<false>(static_cast<const Tables&>(t), texel);
+    return PRE ? f4mul(v, 0.25f) : v;
   }
 
   // srgbFromLinear (preamble:110-121), exact: the bucket of x's top bits gives the

@@ -1,6 +1,6 @@
 // Work-group bodies of the three pipelines, shared by the per-step kernels and
 // the batched kernel.  Each body does one unit of a step ("group"): a tile group
-// of a fast step, 256 outputs or a 16x8 (8x8) tile of a general step, or a
+// of a fast step, 256 outputs or a 31x7 (31x5) tile of a general step, or a
 // whole tail plan.  They take the LDS decode/encode tables (`tab`, any table type the
 // functor's load/store accept) and the LDS scratch they need as pointers, so a
 // kernel can overlay the scratch of different bodies.  Every thread of the
@@ -203,19 +203,23 @@ __device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2
 // ---------------------------------------------------------------------------
 
 // Geometry of a two-level general group: a TX x TY tile of the second level,
-// whose first level (up to (2 TX + 1) x (2 TY + 1) texels, sharedLevel_ of
-// nvpro_pyramid.glsl:555) is computed into LDS by threads that each own a
-// vertical run of R first-level texels of one column.  Narrow texels take the
-// wide tile; 16-byte texels a smaller one to bound registers.
+// whose first level (up to MW x MH texels, sharedLevel_ of
+// nvpro_pyramid.glsl:555) is computed into LDS.  The layout follows the wave:
+// lane c of every wave owns first-level column c of the tile (MW = 63, so a
+// wave spans the tile's width and lane 63 is a helper), and wave w owns the
+// first-level rows [w R, w R + R).  Narrow texels take 7 second-level rows;
+// 16-byte texels 5, to bound the registers of a lane's input window.
 template <uint32_t BYTES>
 struct Gen2Geom
 {
-  static constexpr uint32_t TX = BYTES <= 8u ? 16u : 8u;
-  static constexpr uint32_t TY = 8u;
-  static constexpr uint32_t R  = BYTES <= 8u ? 3u : 2u;
+  static constexpr uint32_t TX = 31;
+  static constexpr uint32_t TY = BYTES <= 8u ? 7u : 5u;
+  static constexpr uint32_t R  = BYTES <= 8u ? 4u : 3u;
   static constexpr uint32_t MW = 2u * TX + 1u;  // LDS row stride of the first level
   static constexpr uint32_t MH = 2u * TY + 1u;
-  static_assert(MW * ((MH + R - 1u) / R) <= (uint32_t)kGroupBS, "one pass of the workgroup");
+  static_assert(MW == 63u, "one lane per first-level column, one wave across the tile");
+  static_assert(4u * R >= MH, "the workgroup's four waves cover the tile's first-level rows");
+  static_assert(TX * TY <= (uint32_t)kGroupBS, "one pass of the workgroup for the second level");
 };
 
 // Groups of a one-level general step (a thread per output) and of a two-level
@@ -263,45 +267,78 @@ constexpr uint32_t gen2MidTexels()
 }
 
 // Two levels: the Gen2Geom tile `tile` of level inLevel+2 and the first-level
-// texels it needs.  First level (fillIntermediateTile_, glsl:732-781): thread t
-// owns column t % MW of the first-level tile, rows R (t / MW) .. + R-1.
-// gen2Load issues the loads of the thread's whole input window (up to 2R+1
-// rows x 3 columns) into a Gen2Window; gen2Compute then decodes each input
-// texel once and computes the run top to bottom (reduceStoreSample_'s order:
-// vertical per column, then horizontal), storing each value and keeping it in
-// `mid` ([y][x], stride MW).  Second level (fillLastTile_, glsl:791-820) from
-// LDS, a thread per texel.  Neighbouring tiles recompute their shared
-// first-level border texels and store identical values.  (Keeping a second
-// window in flight across tiles costs two waves of occupancy per SIMD and
-// measured slower.)
+// texels it needs.
+//
+// First level (fillIntermediateTile_, glsl:732-781): a lane computes the
+// texels of its column in its wave's rows, top to bottom.  Every first-level
+// texel is the reference's reduceStoreSample_ (glsl:575-656): a vertical 3-tap
+// (or 2-tap) reduction of each of its input columns 2c, 2c+1, 2c+2, then a
+// horizontal one.  The vertical reduction of column 2c+2 is the same value as
+// that of the next lane's column 2(c+1), so a lane computes only columns 2c
+// and 2c+1 and takes the third from lane c+1 with a wave shift (DPP
+// wave_shl:1) -- the same operands in the same order, so the same value.  A
+// lane thus loads and decodes each input texel of its two columns once: the
+// thread's input window (Gen2Window, up to 2R+1 rows x 2 columns) is loaded by
+// gen2Load before any arithmetic.  The values go to memory and to `mid`
+// ([y][x], stride MW).
+//
+// Second level (fillLastTile_, glsl:791-820): from LDS, a thread per texel.
+// Neighbouring tiles recompute their shared first-level border texels and
+// store identical values.
 template <class F>
 struct Gen2Window
 {
-  typename F::Raw raw[2u * Gen2Geom<F::kBytes>::R + 1u][3];
+  typename F::Raw raw[2u * Gen2Geom<F::kBytes>::R + 1u][2];
 };
 
-struct Gen2Run
+// Lane i receives lane i+1's value (DPP wave_shl:1; lane 63 keeps its own).
+template <class V>
+__device__ __forceinline__ V waveShiftDown1(V v)
 {
-  uint32_t gx, gy0, n;  // first-level column, first row, outputs of the run (0: idle thread)
+  static_assert(sizeof(V) % 4 == 0, "32-bit words");
+  uint32_t w[sizeof(V) / 4];
+  __builtin_memcpy(w, &v, sizeof v);
+#pragma unroll
+  for(int k = 0; k < (int)(sizeof(V) / 4); ++k)
+    w[k] = (uint32_t)__builtin_amdgcn_update_dpp((int)w[k], (int)w[k], 0x130, 0xf, 0xf, false);
+  V r;
+  __builtin_memcpy(&r, w, sizeof r);
+  return r;
+}
+
+struct Gen2Lane
+{
+  uint32_t gx, gy0, n;  // first-level column, first row of the wave's rows, rows (the same in the whole wave)
+  bool out;             // the lane computes first-level texels (else it only lends column 2 gx)
+  bool c0, c1;          // input columns 2 gx and 2 gx + 1 are loaded
 };
 
 template <class F>
-__device__ __forceinline__ Gen2Run gen2Run(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile)
+__device__ __forceinline__ Gen2Lane gen2Lane(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile)
 {
   using G              = Gen2Geom<F::kBytes>;
+  const LevelDesc& src = img.lv[inLevel];
   const LevelDesc& l1  = img.lv[inLevel + 1u];
+  const int kx1 = kernelSize(src.width);
   const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
-  const uint32_t nx = 2u * G::TX + (kx2 == 3 ? 1u : 0u), ny = 2u * G::TY + (ky2 == 3 ? 1u : 0u);
+  const uint32_t lane = threadIdx.x & 63u, ly0 = (threadIdx.x >> 6) * G::R;
   const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
-  const uint32_t lx = threadIdx.x % G::MW, ly0 = (threadIdx.x / G::MW) * G::R;
-  Gen2Run run{2u * G::TX * tx + lx, 2u * G::TY * ty + ly0, 0u};
-  if(lx < nx && ly0 < ny && run.gx < l1.width && run.gy0 < l1.height)
+  // First-level texels the tile needs (its second level's 2x / 3x windows).
+  const uint32_t nx = 2u * G::TX + (kx2 == 3 ? 1u : 0u), ny = 2u * G::TY + (ky2 == 3 ? 1u : 0u);
+  Gen2Lane r;
+  r.gx  = 2u * G::TX * tx + lane;
+  r.gy0 = 2u * G::TY * ty + ly0;
+  r.n   = 0u;
+  if(ly0 < ny && r.gy0 < l1.height)
   {
     uint32_t n = G::R;
-    n      = ny - ly0 < n ? ny - ly0 : n;
-    run.n  = l1.height - run.gy0 < n ? l1.height - run.gy0 : n;
+    n          = ny - ly0 < n ? ny - ly0 : n;
+    r.n        = l1.height - r.gy0 < n ? l1.height - r.gy0 : n;
   }
-  return run;
+  r.out = lane < nx && r.gx < l1.width;
+  r.c0  = lane <= nx && 2u * r.gx < src.width;  // own column, or lent to lane - 1
+  r.c1  = r.out && kx1 >= 2;
+  return r;
 }
 
 template <class F>
@@ -309,19 +346,29 @@ __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel,
                                          Gen2Window<F>& w)
 {
   constexpr uint32_t NIN = 2u * Gen2Geom<F::kBytes>::R + 1u;
-  const Gen2Run run      = gen2Run<F>(img, inLevel, tilesX, tile);
-  if(run.n == 0u)
-    return;
-  const LevelDesc& src = img.lv[inLevel];
-  const int kx1 = kernelSize(src.width), ky1 = kernelSize(src.height);
-  const uint32_t rows = 2u * (run.n - 1u) + (uint32_t)ky1;  // input rows the run reads
-  const uint8_t* p = img.base + src.offset + (uint64_t)(2u * run.gy0) * src.pitch + (uint64_t)(2u * run.gx) * F::kBytes;
 #pragma unroll
   for(uint32_t r = 0; r < NIN; ++r)
+  {
+    w.raw[r][0] = typename F::Raw{};
+    w.raw[r][1] = typename F::Raw{};
+  }
+  const Gen2Lane L = gen2Lane<F>(img, inLevel, tilesX, tile);
+  if(L.n == 0u)
+    return;
+  const LevelDesc& src = img.lv[inLevel];
+  const uint32_t rows  = 2u * (L.n - 1u) + (uint32_t)kernelSize(src.height);  // input rows the lane reads
+  const uint8_t* p = img.base + src.offset + (uint64_t)(2u * L.gy0) * src.pitch + (uint64_t)(2u * L.gx) * F::kBytes;
 #pragma unroll
-    for(int j = 0; j < 3; ++j)
-      if(r < rows && j < kx1)
-        w.raw[r][j] = loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)j * F::kBytes);
+  for(uint32_t r = 0; r < NIN; ++r)
+  {
+    if(r < rows)
+    {
+      if(L.c0)
+        w.raw[r][0] = loadTexel<F>(p + (uint64_t)r * src.pitch);
+      if(L.c1)
+        w.raw[r][1] = loadTexel<F>(p + (uint64_t)r * src.pitch + F::kBytes);
+    }
+  }
 }
 
 // Contains barriers.
@@ -336,39 +383,39 @@ __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid,
   const LevelDesc& l2  = img.lv[inLevel + 2u];
   const int kx1 = kernelSize(src.width), ky1 = kernelSize(src.height);
   const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
-  const Gen2Run run = gen2Run<F>(img, inLevel, tilesX, tile);
-  if(run.n != 0u)
+  const Gen2Lane L = gen2Lane<F>(img, inLevel, tilesX, tile);
+  if(L.n != 0u)  // the same in the whole wave: the wave shift below sees every lane
   {
     const Axis3 ax = axis3(l1.width), ay = axis3(l1.height);
-    const uint32_t lx = threadIdx.x % G::MW, ly0 = (threadIdx.x / G::MW) * G::R;
-    V top[3];  // decoded input row 2k of output k (the last row of output k-1 when 3-tall)
+    const uint32_t lane = threadIdx.x & 63u, ly0 = (threadIdx.x >> 6) * G::R;
+    V top[2];  // decoded input row 2k of output k (the last row of output k-1 when 3-tall)
 #pragma unroll
-    for(int j = 0; j < 3; ++j)
-      if(j < kx1)
-        top[j] = F::template load<false>(tab, w.raw[0][j]);
+    for(int j = 0; j < 2; ++j)
+      top[j] = F::template load<false>(tab, w.raw[0][j]);
 #pragma unroll
     for(uint32_t k = 0; k < G::R; ++k)
     {
-      if(k < run.n)
+      if(k < L.n)
       {
-        const Weights3 wy = weights3(ay, run.gy0 + k);
-        V h[3];
+        const Weights3 wy = weights3(ay, L.gy0 + k);
+        V h[2];
 #pragma unroll
-        for(int j = 0; j < 3; ++j)
+        for(int j = 0; j < 2; ++j)
         {
-          if(j < kx1)
-          {
-            const V a0 = top[j];
-            const V a1 = ky1 >= 2 ? F::template load<false>(tab, w.raw[2u * k + 1u][j]) : a0;
-            const V a2 = ky1 == 3 ? F::template load<false>(tab, w.raw[2u * k + 2u][j]) : a0;
-            h[j]       = verticalTap<F>(wy, ky1, a0, a1, a2);
-            if(k + 1u < run.n)  // the next output's first row: shared (3-tall) or the row after (2-tall)
-              top[j] = ky1 == 3 ? a2 : F::template load<false>(tab, w.raw[2u * k + 2u][j]);
-          }
+          const V a0 = top[j];
+          const V a1 = ky1 >= 2 ? F::template load<false>(tab, w.raw[2u * k + 1u][j]) : a0;
+          const V a2 = ky1 == 3 ? F::template load<false>(tab, w.raw[2u * k + 2u][j]) : a0;
+          h[j]       = verticalTap<F>(wy, ky1, a0, a1, a2);
+          if(k + 1u < L.n)  // the next output's first row: shared (3-tall) or the row after (2-tall)
+            top[j] = ky1 == 3 ? a2 : F::template load<false>(tab, w.raw[2u * k + 2u][j]);
         }
-        const V v = horizontalTap<F>(ax, run.gx, kx1, h[0], h[1], h[2]);
-        storeTexel<F>(texelPtr(img, inLevel + 1u, run.gx, run.gy0 + k, F::kBytes), F::store(tab, v));
-        mid[(ly0 + k) * G::MW + lx] = v;
+        const V h2 = waveShiftDown1(h[0]);  // column 2 gx + 2, from lane + 1
+        if(L.out)
+        {
+          const V v = horizontalTap<F>(ax, L.gx, kx1, h[0], h[1], h2);
+          storeTexel<F>(texelPtr(img, inLevel + 1u, L.gx, L.gy0 + k, F::kBytes), F::store(tab, v));
+          mid[(ly0 + k) * G::MW + lane] = v;
+        }
       }
     }
   }
@@ -412,13 +459,51 @@ __device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* 
 // least the first output level's texel count.  BS threads (the workgroup).
 // Contains barriers.
 // ---------------------------------------------------------------------------
-//
+
+// The (up to) 3x3 input window of output i of level dl, from level sl.
+template <class F>
+struct TailWindow
+{
+  typename F::Raw raw[3][3];
+};
+
+template <class F>
+__device__ __forceinline__ void tailLoad(const ImageDesc& img, const LevelDesc& sl, const LevelDesc& dl, uint32_t i,
+                                         TailWindow<F>& w)
+{
+  const int kx = kernelSize(sl.width), ky = kernelSize(sl.height);
+  const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
+  const uint8_t* q  = img.base + sl.offset + (uint64_t)(2u * dy) * sl.pitch + (uint64_t)(2u * dx) * F::kBytes;
+#pragma unroll
+  for(int k = 0; k < 3; ++k)
+#pragma unroll
+    for(int j = 0; j < 3; ++j)
+      if(j < kx && k < ky)
+        w.raw[k][j] = loadTexel<F>(q + (uint64_t)k * sl.pitch + (uint64_t)j * F::kBytes);
+}
+
+// This thread's first window of a plan (output threadIdx.x of its first
+// level), loaded ahead of tailRun: a kernel issues it before filling its
+// tables, so the two memory latencies overlap.
+template <class F>
+__device__ __forceinline__ void tailPrefetch(const ImageDesc& img, const TailPlan& plan, TailWindow<F>& w)
+{
+  if(plan.count == 0u)
+    return;
+  const LevelDesc& sl = img.lv[plan.steps[0].inputLevel];
+  const LevelDesc& dl = img.lv[plan.steps[0].inputLevel + 1u];
+  if(threadIdx.x < dl.width * dl.height)
+    tailLoad<F>(img, sl, dl, threadIdx.x, w);
+}
+
 // The first level reads memory: its outputs go in chunks of PF passes of the
 // workgroup, every load of a chunk issued before its arithmetic, so a chunk
 // pays one memory latency instead of one per pass (9 PF texels of registers).
+// `pre`: this thread's first window from tailPrefetch, or null.
 template <class F, int BS, int PF = 1, class TT>
 __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, typename F::Val* buf1,
-                                        const ImageDesc& img, const TailPlan& plan)
+                                        const ImageDesc& img, const TailPlan& plan,
+                                        const TailWindow<F>* pre = nullptr)
 {
   using V      = typename F::Val;
   using Raw    = typename F::Raw;
@@ -448,22 +533,15 @@ __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, ty
       {
         for(uint32_t base = 0; base < n; base += PF * BS)
         {
-          Raw raw[PF][3][3];
+          TailWindow<F> win[PF];
 #pragma unroll
           for(int p = 0; p < PF; ++p)
           {
             const uint32_t i = base + (uint32_t)p * BS + threadIdx.x;
-            if(i < n)
-            {
-              const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
-              const uint8_t* q  = img.base + sl.offset + (uint64_t)(2u * dy) * sl.pitch + (uint64_t)(2u * dx) * F::kBytes;
-#pragma unroll
-              for(int k = 0; k < 3; ++k)
-#pragma unroll
-                for(int j = 0; j < 3; ++j)
-                  if(j < kx && k < ky)
-                    raw[p][k][j] = loadTexel<F>(q + (uint64_t)k * sl.pitch + (uint64_t)j * F::kBytes);
-            }
+            if(pre && base == 0u && p == 0)
+              win[0] = *pre;
+            else if(i < n)
+              tailLoad<F>(img, sl, dl, i, win[p]);
           }
 #pragma unroll
           for(int p = 0; p < PF; ++p)
@@ -472,7 +550,7 @@ __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, ty
             if(i < n)
             {
               const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
-              auto at = [&](int j, int k) -> V { return F::template load<false>(tab, raw[p][k][j]); };
+              auto at = [&](int j, int k) -> V { return F::template load<false>(tab, win[p].raw[k][j]); };
               finish(i, dx, dy, fast ? box<F, false>(at(0, 0), at(1, 0), at(0, 1), at(1, 1), order)
                                      : generalSample<F>(at, kx, ky, ax, ay, dx, dy));
             }

@@ -35,8 +35,9 @@ uint64_t groupInputBytes(uint32_t kind, uint32_t bpt)
     return (uint64_t)kGroupBS * 16u * bpt;  // 4x4 per lane
   if(kind == kSegGeneral1)
     return (uint64_t)kGroupBS * 4u * bpt;
-  if(kind == kSegGeneral2)
-    return bpt <= 8u ? 67ull * 35ull * bpt : 35ull * 35ull * bpt;
+  if(kind == kSegGeneral2)  // the input under a tile's first level
+    return bpt <= 8u ? (2ull * Gen2Geom<8>::MW + 1u) * (2ull * Gen2Geom<8>::MH + 1u) * bpt
+                     : (2ull * Gen2Geom<16>::MW + 1u) * (2ull * Gen2Geom<16>::MH + 1u) * bpt;
   return 1u;
 }
 
