@@ -12,15 +12,15 @@
 //
 // Queue mode: workgroups are persistent and take the next item from an atomic
 // head counter.  Before an item of step s a workgroup waits for the completion
-// counter of step s-1 to reach its item count; after an item of a step with a
-// successor it publishes its stores (agent-scope release by every wave) and
-// bumps the step's counter.  Items are taken in list order, so the unfinished
-// item with the smallest index never waits on anything unfinished: the launch
-// completes whatever the number of resident workgroups.  Each wait is bounded
-// all the same (spinLimit); a wait that gives up sets an error flag the host can
-// read, and the workgroup carries on, so a launch always ends.  The last
-// workgroup to exit clears the head, exit and completion counters for the next
-// launch.
+// counter of step s-1 to reach its item count (then drops stale cache lines);
+// an item stores its step's last level with write-through stores, waits for
+// them to complete and bumps the step's counter.  Items are taken in list
+// order, so the unfinished item with the smallest index never waits on
+// anything unfinished: the launch completes whatever the number of resident
+// workgroups.  Each wait is bounded all the same (spinLimit); a wait that gives
+// up sets an error flag the host can read, and the workgroup carries on, so a
+// launch always ends.  The last workgroup to exit (arriveLast) clears the head
+// and completion counters for the next launch.
 //
 // Rounds mode: the items of one round are independent; workgroups take items
 // blockIdx.x, blockIdx.x + gridDim.x, ... and stream order separates rounds.
@@ -67,7 +67,9 @@ __device__ __forceinline__ void waitCounter(uint32_t* counter, uint32_t need, ui
       const TailPlan *__restrict__ tails, uint32_t *__restrict__ counters, uint32_t itemBegin, uint32_t itemEnd,    \
       uint32_t numSlots, uint32_t spinLimit
 
-template <class F>
+// CO: the step's last level is stored coherently, for the queue mode's
+// successors in the same launch.
+template <class F, bool CO>
 __device__ __forceinline__ void runItem(BatchShared<F>& sh, const ImageDesc* __restrict__ imgs,
                                         const TailPlan* __restrict__ tails, const QueueSeg& seg,
                                         const QueueItem& item, uint32_t& iter)
@@ -80,14 +82,14 @@ __device__ __forceinline__ void runItem(BatchShared<F>& sh, const ImageDesc* __r
   {
     switch(seg.kind)
     {
-      case 1: fastGroup<F, 1>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 2: fastGroup<F, 2>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 3: fastGroup<F, 3>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 4: fastGroup<F, 4>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 5: fastGroup<F, 5>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 6: fastGroup<F, 6>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case kSegGeneral1: generalOneGroup<F>(sh.tab, img, seg.inputLevel, g); break;
-      case kSegGeneral2: generalTwoGroup<F>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, g); break;
+      case 1: fastGroup<F, 1, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 2: fastGroup<F, 2, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 3: fastGroup<F, 3, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 4: fastGroup<F, 4, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 5: fastGroup<F, 5, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 6: fastGroup<F, 6, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case kSegGeneral1: generalOneGroup<F, CO>(sh.tab, img, seg.inputLevel, g); break;
+      case kSegGeneral2: generalTwoGroup<F, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, g); break;
       case kSegTail:
         tailRun<F, kGroupBS>(sh.tab, scratch, scratch + kBatchTailTexels, img, tails[seg.inputLevel]);
         break;
@@ -108,7 +110,7 @@ batchRoundKernel(NVPRO_BATCH_PARAMS)
   {
     const QueueItem item = items[it];
     const QueueSeg seg   = segs[item.seg];
-    runItem<F>(sh, imgs, tails, seg, item, iter);
+    runItem<F, false>(sh, imgs, tails, seg, item, iter);
     __syncthreads();  // the next item may overlay this one's scratch
   }
 }
@@ -118,7 +120,7 @@ __global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(F
 batchQueueKernel(NVPRO_BATCH_PARAMS)
 {
   __shared__ BatchShared<F> sh;
-  __shared__ uint32_t sNext, sLast;
+  __shared__ uint32_t sNext, sFlag;
   uint32_t* head  = counters + kCounterHead;
   uint32_t* error = counters + kCounterError;
   uint32_t* slots = counters + kCounterSlots;
@@ -145,39 +147,33 @@ batchQueueKernel(NVPRO_BATCH_PARAMS)
     if(seg.depSlot != kNoSlot)
     {
       __syncthreads();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // see the previous step's stores
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // drop stale lines before reading the previous step
     }
-    runItem<F>(sh, imgs, tails, seg, item, iter);
+    runItem<F, true>(sh, imgs, tails, seg, item, iter);
+    // Publish: the step's last level went out with write-through stores; once
+    // every wave's stores have completed the item is counted.  (An agent-scope
+    // release fence here would write back the whole L2 of the XCD per wave and
+    // item, which costs milliseconds over a batch.)
     if(seg.doneSlot != kNoSlot)
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // publish this wave's stores
+      __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if(threadIdx.x == 0)
     {
       if(seg.doneSlot != kNoSlot)
-        __hip_atomic_fetch_add(slots + seg.doneSlot, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(slots + seg.doneSlot, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       sNext = next;
     }
     __syncthreads();
     it = __builtin_amdgcn_readfirstlane(sNext);
   }
   // Exit: the last workgroup out resets the counters for the next launch.  Every
-  // other workgroup has made its last head, slot and exit update by then.
-  if(threadIdx.x == 0)
-  {
-    const uint32_t done =
-        __hip_atomic_fetch_add(counters + kCounterExited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    sLast = done + 1u == gridDim.x ? 1u : 0u;
-  }
-  __syncthreads();
-  if(sLast)
+  // other workgroup has made its last head and slot update by then.
+  if(arriveLast(counters + batchExitCounters(numSlots), gridDim.x, &sFlag))
   {
     for(uint32_t i = threadIdx.x; i < numSlots; i += kGroupBS)
       __hip_atomic_store(slots + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if(threadIdx.x == 0)
-    {
       __hip_atomic_store(head, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(counters + kCounterExited, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
   }
 }
 

@@ -241,7 +241,7 @@ __host__ __device__ inline uint32_t generalGroupCount(uint32_t dstWidth, uint32_
 
 // One level (glsl:828-842): outputs [256 g, 256 g + 256) of level inLevel+1, in
 // row-major order, a thread each.  No barrier.
-template <class F, class TT>
+template <class F, bool CO = false, class TT>
 __device__ __forceinline__ void generalOneGroup(const TT& tab, const ImageDesc& img, uint32_t inLevel, uint32_t g)
 {
   const LevelDesc& src = img.lv[inLevel];
@@ -256,7 +256,7 @@ __device__ __forceinline__ void generalOneGroup(const TT& tab, const ImageDesc& 
     return F::template load<false>(tab, loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)j * F::kBytes));
   };
   const typename F::Val v = generalSample<F>(at, kx, ky, axis3(dst.width), axis3(dst.height), dx, dy);
-  storeTexel<F>(texelPtr(img, inLevel + 1u, dx, dy, F::kBytes), F::store(tab, v));
+  storeTexelTo<F, CO>(texelPtr(img, inLevel + 1u, dx, dy, F::kBytes), F::store(tab, v));
 }
 
 // LDS values of a two-level general group.
@@ -371,8 +371,8 @@ __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel,
   }
 }
 
-// Contains barriers.
-template <class F, class TT>
+// Contains barriers.  CO: the second level is stored with storeTexelCoherent.
+template <class F, bool CO = false, class TT>
 __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid, const ImageDesc& img,
                                             uint32_t inLevel, uint32_t tilesX, uint32_t tile, const Gen2Window<F>& w)
 {
@@ -429,20 +429,20 @@ __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid,
     {
       auto at = [&](int j, int r) { return mid[(2u * oy + r) * G::MW + 2u * ox + j]; };
       const V v = generalSample<F>(at, kx2, ky2, axis3(l2.width), axis3(l2.height), x, y);
-      storeTexel<F>(texelPtr(img, inLevel + 2u, x, y, F::kBytes), F::store(tab, v));
+      storeTexelTo<F, CO>(texelPtr(img, inLevel + 2u, x, y, F::kBytes), F::store(tab, v));
     }
   }
   __syncthreads();
 }
 
 // One tile.  Contains barriers.
-template <class F, class TT>
+template <class F, bool CO = false, class TT>
 __device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* mid, const ImageDesc& img,
                                                 uint32_t inLevel, uint32_t tilesX, uint32_t tile)
 {
   Gen2Window<F> w;
   gen2Load<F>(img, inLevel, tilesX, tile, w);
-  gen2Compute<F>(tab, mid, img, inLevel, tilesX, tile, w);
+  gen2Compute<F, CO>(tab, mid, img, inLevel, tilesX, tile, w);
 }
 
 // ---------------------------------------------------------------------------

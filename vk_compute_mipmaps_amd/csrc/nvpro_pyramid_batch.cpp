@@ -203,7 +203,7 @@ hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& p
   const size_t oItm = align256(oSeg + p.segs.size() * sizeof(QueueSeg));
   const size_t oTl  = align256(oItm + p.items.size() * sizeof(QueueItem));
   const size_t oCnt = align256(oTl + p.tails.size() * sizeof(TailPlan));
-  const size_t size = align256(oCnt + (kCounterSlots + p.numSlots) * sizeof(uint32_t));
+  const size_t size = align256(oCnt + batchCounterWords(p.numSlots) * sizeof(uint32_t));
   if(size > devCap_)
   {
     // Grow (the previous buffer is idle: the caller staged nothing in flight).

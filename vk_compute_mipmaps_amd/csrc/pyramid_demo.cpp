@@ -15,6 +15,7 @@
 //       checks that both write identical bytes, prints one JSON object.
 //   nvpro_pyramid_demo serve [--rate R] [--count N] [--seed S] [--hold-ms H]
 //                            [--budget-gb B] [--in-flight K] [--max-batch M] [--packed]
+//                            [--queue | --rounds]
 //       Serving loop: N requests arrive as a Poisson stream of R per second;
 //       sizes from the mixed-size workload, 70% sRGBA8 / 30% RGBA16F; level 0
 //       is the seeded synthetic image, written on the device at launch.  The
@@ -297,6 +298,7 @@ struct ServeArgs
   uint32_t count = 20000, inFlight = 4, maxBatch = 0;
   uint64_t seed = 9;
   bool packed = false;
+  int mode    = -1;  // batch mode; -1: the server's default
 };
 
 static int serve(const ServeArgs& a)
@@ -321,6 +323,8 @@ static int serve(const ServeArgs& a)
   if(a.maxBatch)
     cfg.maxBatchImages = a.maxBatch;
   cfg.arithmetic16f = a.packed ? NVPRO_PYRAMID_ARITH_PACKED_FP16 : NVPRO_PYRAMID_ARITH_FP32;
+  if(a.mode >= 0)
+    cfg.batch.mode = (uint32_t)a.mode;
   NvproPyramidServer* srv = nullptr;
   CHECK_PYR(nvproPyramidServerCreate(&cfg, &srv));
   std::vector<NvproPyramidCompletion> done(4096);
@@ -370,9 +374,9 @@ static int serve(const ServeArgs& a)
   CHECK_PYR(nvproPyramidServerDestroy(srv));
   const double span = lastDone;
   printf("{\"requests\": %u, \"rate_per_s\": %.1f, \"mix\": \"70%% sRGBA8 / 30%% RGBA16F (%s)\", "
-         "\"budget_bytes\": %llu, \"in_flight\": %u, \"hold_ms\": %.1f,\n",
+         "\"budget_bytes\": %llu, \"in_flight\": %u, \"hold_ms\": %.1f, \"batch_mode\": \"%s\",\n",
          a.count, a.rate, a.packed ? "packed_fp16" : "fp32", (unsigned long long)cfg.budgetBytes, a.inFlight,
-         a.holdMs);
+         a.holdMs, cfg.batch.mode == NVPRO_PYRAMID_BATCH_MODE_QUEUE ? "queue" : "rounds");
   printf(" \"seconds\": %.4f, \"chains_per_s\": %.1f, \"GBps\": %.1f, \"failed\": %u,\n", span,
          a.count / span, bytes / span / 1e9, failed);
   printf(" \"latency_us\": {\"p50\": %.1f, \"p90\": %.1f, \"p99\": %.1f, \"max\": %.1f}, "
@@ -410,6 +414,10 @@ int main(int argc, char** argv)
         a.maxBatch = (uint32_t)atoi(argv[++i]);
       else if(o == "--packed")
         a.packed = true;
+      else if(o == "--queue")
+        a.mode = NVPRO_PYRAMID_BATCH_MODE_QUEUE;
+      else if(o == "--rounds")
+        a.mode = NVPRO_PYRAMID_BATCH_MODE_ROUNDS;
       else
       {
         fprintf(stderr, "unknown option %s\n", o.c_str());

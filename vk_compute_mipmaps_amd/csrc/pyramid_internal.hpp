@@ -65,9 +65,10 @@ struct QueueItem
 {
   uint32_t seg, firstGroup, groupCount, pad;
 };
-// Counter block of a batch: head of the queue, workgroups exited, sticky error
-// flags, then one completion counter per slot.
-constexpr uint32_t kCounterHead = 0, kCounterExited = 1, kCounterError = 2, kCounterSlots = 4;
+// Counter block of a batch: head of the queue, (unused), sticky error flags,
+// then one completion counter per slot, then the arrival counters
+// (kArriveWords, see below) of the queue kernel's exit.
+constexpr uint32_t kCounterHead = 0, kCounterError = 2, kCounterSlots = 4;
 constexpr uint32_t kBatchErrorSpin = 1u;  // a dependency wait gave up
 // Largest first output level a tail may have inside the batched kernel (its LDS
 // ping-pong overlays the other bodies' scratch).
@@ -84,6 +85,17 @@ constexpr uint32_t kBatchTailTexels = 128;
 constexpr uint32_t kArriveLeaves = 64;
 constexpr uint32_t kArriveStride = 64;  // words between counters
 constexpr uint32_t kArriveWords  = (1u + kArriveLeaves) * kArriveStride;
+
+// Offset of a batch's exit arrival counters in its counter block, and the
+// block's size in words.
+__host__ __device__ inline uint32_t batchExitCounters(uint32_t numSlots)
+{
+  return (kCounterSlots + numSlots + kArriveStride - 1u) / kArriveStride * kArriveStride;
+}
+__host__ __device__ inline uint32_t batchCounterWords(uint32_t numSlots)
+{
+  return batchExitCounters(numSlots) + kArriveWords;
+}
 
 // What a fused launch runs in its last workgroup after its own step: a fast
 // step of one group (kNextFast: inLevel, count, tilesX, numTiles) or the tail
