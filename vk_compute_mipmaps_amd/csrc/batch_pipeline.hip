@@ -46,20 +46,6 @@ struct BatchShared
   alignas(16) unsigned char scratch[BatchScratch<F>::kBytes];
 };
 
-__device__ __forceinline__ void waitCounter(uint32_t* counter, uint32_t need, uint32_t spinLimit, uint32_t* error)
-{
-  uint32_t n = 0;
-  while(__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need)
-  {
-    if(++n >= spinLimit)
-    {
-      __hip_atomic_fetch_or(error, kBatchErrorSpin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      break;
-    }
-    __builtin_amdgcn_s_sleep(2);
-  }
-}
-
 // The work list is read-only during a launch and never aliases texel memory;
 // the restrict-qualified pointers let its fields live in scalar registers.
 #define NVPRO_BATCH_PARAMS                                                                                          \

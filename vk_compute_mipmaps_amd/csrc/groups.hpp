@@ -482,28 +482,12 @@ __device__ __forceinline__ void tailLoad(const ImageDesc& img, const LevelDesc& 
         w.raw[k][j] = loadTexel<F>(q + (uint64_t)k * sl.pitch + (uint64_t)j * F::kBytes);
 }
 
-// This thread's first window of a plan (output threadIdx.x of its first
-// level), loaded ahead of tailRun: a kernel issues it before filling its
-// tables, so the two memory latencies overlap.
-template <class F>
-__device__ __forceinline__ void tailPrefetch(const ImageDesc& img, const TailPlan& plan, TailWindow<F>& w)
-{
-  if(plan.count == 0u)
-    return;
-  const LevelDesc& sl = img.lv[plan.steps[0].inputLevel];
-  const LevelDesc& dl = img.lv[plan.steps[0].inputLevel + 1u];
-  if(threadIdx.x < dl.width * dl.height)
-    tailLoad<F>(img, sl, dl, threadIdx.x, w);
-}
-
 // The first level reads memory: its outputs go in chunks of PF passes of the
 // workgroup, every load of a chunk issued before its arithmetic, so a chunk
 // pays one memory latency instead of one per pass (9 PF texels of registers).
-// `pre`: this thread's first window from tailPrefetch, or null.
 template <class F, int BS, int PF = 1, class TT>
 __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, typename F::Val* buf1,
-                                        const ImageDesc& img, const TailPlan& plan,
-                                        const TailWindow<F>* pre = nullptr)
+                                        const ImageDesc& img, const TailPlan& plan)
 {
   using V      = typename F::Val;
   using Raw    = typename F::Raw;
@@ -538,9 +522,7 @@ __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, ty
           for(int p = 0; p < PF; ++p)
           {
             const uint32_t i = base + (uint32_t)p * BS + threadIdx.x;
-            if(pre && base == 0u && p == 0)
-              win[0] = *pre;
-            else if(i < n)
+            if(i < n)
               tailLoad<F>(img, sl, dl, i, win[p]);
           }
 #pragma unroll

@@ -181,6 +181,23 @@ __device__ __forceinline__ bool arriveLast(uint32_t* counters, uint32_t groups, 
   return last;
 }
 
+// Waits (one thread) until an agent-scope counter reaches `need`.  Bounded: after
+// spinLimit polls it sets kBatchErrorSpin in *error and returns, so a kernel
+// always ends.
+__device__ __forceinline__ void waitCounter(uint32_t* counter, uint32_t need, uint32_t spinLimit, uint32_t* error)
+{
+  uint32_t n = 0;
+  while(__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need)
+  {
+    if(++n >= spinLimit)
+    {
+      __hip_atomic_fetch_or(error, kBatchErrorSpin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+}
+
 // Even bits of v packed together (Morton decode of one coordinate).
 __device__ __forceinline__ uint32_t compactEven(uint32_t v)
 {

@@ -18,11 +18,9 @@ template <class F, int MAXT>
 __global__ void __launch_bounds__(kTailBS) tailKernel(ImageDesc img, TailPlan plan)
 {
   __shared__ TailShared<F, MAXT> sh;
-  TailWindow<F> first;
-  tailPrefetch<F>(img, plan, first);  // in flight during the table fill
   F::template fill<kTailBS>(sh.tab, 1.0f);
   __syncthreads();
-  tailRun<F, kTailBS>(sh.tab, sh.buf[0], sh.buf[1], img, plan, &first);
+  tailRun<F, kTailBS>(sh.tab, sh.buf[0], sh.buf[1], img, plan);
 }
 
 template <class F>
