@@ -204,26 +204,31 @@ hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& p
   const size_t oTl  = align256(oItm + p.items.size() * sizeof(QueueItem));
   const size_t oCnt = align256(oTl + p.tails.size() * sizeof(TailPlan));
   const size_t size = align256(oCnt + batchCounterWords(p.numSlots) * sizeof(uint32_t));
+  // Grow geometrically (the previous buffers are idle: the caller staged
+  // nothing in flight).  A serving loop stages a new work list per batch;
+  // growing to the exact size would reallocate often, and hipFree waits for
+  // the whole device.
   if(size > devCap_)
   {
-    // Grow (the previous buffer is idle: the caller staged nothing in flight).
+    const size_t want = std::max(size, 2u * devCap_);
     if(dev_)
       (void)hipFree(dev_);
     dev_    = nullptr;
     devCap_ = 0;
-    if(hipError_t e = hipMalloc(&dev_, size))
+    if(hipError_t e = hipMalloc(&dev_, want))
       return e;
-    devCap_ = size;
+    devCap_ = want;
   }
   if(size > pinnedCap_)
   {
+    const size_t want = std::max(size, 2u * pinnedCap_);
     if(pinned_)
       (void)hipHostFree(pinned_);
     pinned_    = nullptr;
     pinnedCap_ = 0;
-    if(hipError_t e = hipHostMalloc(&pinned_, size, hipHostMallocDefault))
+    if(hipError_t e = hipHostMalloc(&pinned_, want, hipHostMallocDefault))
       return e;
-    pinnedCap_ = size;
+    pinnedCap_ = want;
   }
   uint8_t* h = (uint8_t*)pinned_;
   memset(h, 0, size);
@@ -305,13 +310,13 @@ hipError_t DeviceStaging::upload(const void* host, size_t bytes, hipStream_t str
 {
   if(bytes > cap_)
   {
+    const size_t want = std::max(align256(bytes), 2u * cap_);  // geometric, as BatchRunner::stage
     if(dev_)
       (void)hipFree(dev_);
     if(pinned_)
       (void)hipHostFree(pinned_);
     dev_ = pinned_ = nullptr;
     cap_           = 0;
-    const size_t want = align256(bytes);
     if(hipError_t e = hipMalloc(&dev_, want))
       return e;
     if(hipError_t e = hipHostMalloc(&pinned_, want, hipHostMallocDefault))
