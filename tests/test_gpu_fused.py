@@ -119,3 +119,29 @@ def test_fused_equals_separate_float(fmt, arith, w, h, stream):
         _check(chain, stream, arithmetic=arith)
     finally:
         chain.free()
+
+
+def test_fused_concurrent_streams_srgba8(stream):
+    """Fused dispatches on two streams in flight at once (each stream has its
+    own arrival counters): every chain still matches its separate-launch bytes."""
+    sizes = [(4096, 4096), (2048, 2048), (8192, 8192)]
+    chains = [P.DeviceMipChain(w, h, P.FORMAT_SRGBA8) for w, h in sizes]
+    other = hip_rt.stream_create()
+    try:
+        want = []
+        for k, c in enumerate(chains):
+            c.synthesize(20 + k, stream)
+            want.append(_run(c, stream, False)[0])
+        streams = [stream, other]
+        for rep in range(6):
+            for k, c in enumerate(chains):
+                c.dispatch(stream=streams[(rep + k) % 2])
+        hip_rt.stream_sync(stream)
+        hip_rt.stream_sync(other)
+        for k, c in enumerate(chains):
+            assert np.array_equal(c.download(), want[k]), f"{sizes[k]}: concurrent fused dispatches differ"
+    finally:
+        hip_rt.stream_sync(other)
+        hip_rt.stream_destroy(other)
+        for c in chains:
+            c.free()
