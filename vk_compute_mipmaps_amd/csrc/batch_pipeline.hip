@@ -75,7 +75,7 @@ __device__ __forceinline__ void runItem(BatchShared<F>& sh, const ImageDesc* __r
       case 5: fastGroup<F, 5, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
       case 6: fastGroup<F, 6, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
       case kSegGeneral1: generalOneGroup<F, CO>(sh.tab, img, seg.inputLevel, g); break;
-      case kSegGeneral2: generalTwoGroup<F, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, g); break;
+      case kSegGeneral2: generalTwoGroupAny<F, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, g); break;
       case kSegTail:
         tailRun<F, kGroupBS>(sh.tab, scratch, scratch + kBatchTailTexels, img, tails[seg.inputLevel]);
         break;

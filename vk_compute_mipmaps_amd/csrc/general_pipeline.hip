@@ -22,7 +22,8 @@ __global__ void __launch_bounds__(kGroupBS) generalOneLevelKernel(ImageDesc img,
     generalOneGroup<F>(tab, img, inLevel, g);
 }
 
-template <class F>
+// KX, KY: the first level's kernel sizes, or 0 (read from the image).
+template <class F, int KX, int KY>
 __global__ void __launch_bounds__(kGroupBS) generalTwoLevelKernel(ImageDesc img, uint32_t inLevel, uint32_t tilesX,
                                                                 uint32_t numTiles)
 {
@@ -30,7 +31,7 @@ __global__ void __launch_bounds__(kGroupBS) generalTwoLevelKernel(ImageDesc img,
   F::template fill<kGroupBS>(sh.tab, 1.0f);
   __syncthreads();
   for(uint32_t tile = blockIdx.x; tile < numTiles; tile += gridDim.x)
-    generalTwoGroup<F>(sh.tab, sh.mid, img, inLevel, tilesX, tile);
+    generalTwoGroup<F, false, KX, KY>(sh.tab, sh.mid, img, inLevel, tilesX, tile);
 }
 
 // Workgroups of `kernel` that are resident on one compute unit at once.  A
@@ -45,22 +46,44 @@ static uint32_t residentPerCU(K kernel)
   return (uint32_t)n;
 }
 
+template <class F, int KX, int KY>
+static void launchTwoLevel(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t tilesX,
+                           uint32_t numTiles)
+{
+  static const uint32_t fit = residentPerCU(generalTwoLevelKernel<F, KX, KY>);
+  const uint64_t cap        = (uint64_t)deviceComputeUnits() * fit;
+  const uint32_t blocks     = (uint32_t)(numTiles < cap ? numTiles : cap);
+  hipLaunchKernelGGL((generalTwoLevelKernel<F, KX, KY>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
+                     inLevel, tilesX, numTiles);
+}
+
 template <class F>
 static hipError_t launchGeneralTyped(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t count)
 {
   static const uint32_t oneFit = residentPerCU(generalOneLevelKernel<F>);
-  static const uint32_t twoFit = residentPerCU(generalTwoLevelKernel<F>);
   const LevelDesc& d           = img.lv[inLevel + count];
   uint32_t tilesX              = 0;
   const uint64_t want          = generalGroupCount(d.width, d.height, count, F::kBytes, &tilesX);
-  const uint64_t cap           = (uint64_t)deviceComputeUnits() * (count == 1u ? oneFit : twoFit);
-  const uint32_t blocks        = (uint32_t)(want < cap ? want : cap);
   if(count == 1u)
+  {
+    const uint64_t cap    = (uint64_t)deviceComputeUnits() * oneFit;
+    const uint32_t blocks = (uint32_t)(want < cap ? want : cap);
     hipLaunchKernelGGL((generalOneLevelKernel<F>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
                        inLevel, (uint32_t)want);
+    return hipGetLastError();
+  }
+  // Two levels: the kernel specialised for the first level's kernel sizes.
+  const int kx = kernelSize(img.lv[inLevel].width), ky = kernelSize(img.lv[inLevel].height);
+  if(kx == 3 && ky == 3)
+    launchTwoLevel<F, 3, 3>(stream, img, inLevel, tilesX, (uint32_t)want);
+  else if(kx == 2 && ky == 2)
+    launchTwoLevel<F, 2, 2>(stream, img, inLevel, tilesX, (uint32_t)want);
+  else if(kx == 3 && ky == 2)
+    launchTwoLevel<F, 3, 2>(stream, img, inLevel, tilesX, (uint32_t)want);
+  else if(kx == 2 && ky == 3)
+    launchTwoLevel<F, 2, 3>(stream, img, inLevel, tilesX, (uint32_t)want);
   else
-    hipLaunchKernelGGL((generalTwoLevelKernel<F>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
-                       inLevel, tilesX, (uint32_t)want);
+    launchTwoLevel<F, 0, 0>(stream, img, inLevel, tilesX, (uint32_t)want);
   return hipGetLastError();
 }
 

@@ -313,13 +313,16 @@ struct Gen2Lane
   bool c0, c1;          // input columns 2 gx and 2 gx + 1 are loaded
 };
 
-template <class F>
+// KX, KY: the first level's kernel sizes (kernelSize of the input width and
+// height) when known at compile time, 0 to read them from the image.  Each
+// specialisation computes exactly what the general one does.
+template <class F, int KX = 0>
 __device__ __forceinline__ Gen2Lane gen2Lane(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile)
 {
   using G              = Gen2Geom<F::kBytes>;
   const LevelDesc& src = img.lv[inLevel];
   const LevelDesc& l1  = img.lv[inLevel + 1u];
-  const int kx1 = kernelSize(src.width);
+  const int kx1 = KX ? KX : kernelSize(src.width);
   const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
   const uint32_t lane = threadIdx.x & 63u, ly0 = (threadIdx.x >> 6) * G::R;
   const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
@@ -341,7 +344,7 @@ __device__ __forceinline__ Gen2Lane gen2Lane(const ImageDesc& img, uint32_t inLe
   return r;
 }
 
-template <class F>
+template <class F, int KX = 0, int KY = 0>
 __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile,
                                          Gen2Window<F>& w)
 {
@@ -352,11 +355,11 @@ __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel,
     w.raw[r][0] = typename F::Raw{};
     w.raw[r][1] = typename F::Raw{};
   }
-  const Gen2Lane L = gen2Lane<F>(img, inLevel, tilesX, tile);
+  const Gen2Lane L = gen2Lane<F, KX>(img, inLevel, tilesX, tile);
   if(L.n == 0u)
     return;
   const LevelDesc& src = img.lv[inLevel];
-  const uint32_t rows  = 2u * (L.n - 1u) + (uint32_t)kernelSize(src.height);  // input rows the lane reads
+  const uint32_t rows  = 2u * (L.n - 1u) + (uint32_t)(KY ? KY : kernelSize(src.height));  // input rows the lane reads
   const uint8_t* p = img.base + src.offset + (uint64_t)(2u * L.gy0) * src.pitch + (uint64_t)(2u * L.gx) * F::kBytes;
 #pragma unroll
   for(uint32_t r = 0; r < NIN; ++r)
@@ -372,7 +375,7 @@ __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel,
 }
 
 // Contains barriers.  CO: the second level is stored with storeTexelCoherent.
-template <class F, bool CO = false, class TT>
+template <class F, bool CO = false, int KX = 0, int KY = 0, class TT>
 __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid, const ImageDesc& img,
                                             uint32_t inLevel, uint32_t tilesX, uint32_t tile, const Gen2Window<F>& w)
 {
@@ -381,9 +384,9 @@ __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid,
   const LevelDesc& src = img.lv[inLevel];
   const LevelDesc& l1  = img.lv[inLevel + 1u];
   const LevelDesc& l2  = img.lv[inLevel + 2u];
-  const int kx1 = kernelSize(src.width), ky1 = kernelSize(src.height);
+  const int kx1 = KX ? KX : kernelSize(src.width), ky1 = KY ? KY : kernelSize(src.height);
   const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
-  const Gen2Lane L = gen2Lane<F>(img, inLevel, tilesX, tile);
+  const Gen2Lane L = gen2Lane<F, KX>(img, inLevel, tilesX, tile);
   if(L.n != 0u)  // the same in the whole wave: the wave shift below sees every lane
   {
     const Axis3 ax = axis3(l1.width), ay = axis3(l1.height);
@@ -436,13 +439,33 @@ __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid,
 }
 
 // One tile.  Contains barriers.
-template <class F, bool CO = false, class TT>
+template <class F, bool CO = false, int KX = 0, int KY = 0, class TT>
 __device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* mid, const ImageDesc& img,
                                                 uint32_t inLevel, uint32_t tilesX, uint32_t tile)
 {
   Gen2Window<F> w;
-  gen2Load<F>(img, inLevel, tilesX, tile, w);
-  gen2Compute<F, CO>(tab, mid, img, inLevel, tilesX, tile, w);
+  gen2Load<F, KX, KY>(img, inLevel, tilesX, tile, w);
+  gen2Compute<F, CO, KX, KY>(tab, mid, img, inLevel, tilesX, tile, w);
+}
+
+// One tile, through the specialisation for the first level's kernel sizes
+// (2 or 3 each; a side of 1 takes the general code).  The choice is the same
+// for every tile of a step.
+template <class F, bool CO = false, class TT>
+__device__ __forceinline__ void generalTwoGroupAny(const TT& tab, typename F::Val* mid, const ImageDesc& img,
+                                                   uint32_t inLevel, uint32_t tilesX, uint32_t tile)
+{
+  const int kx = kernelSize(img.lv[inLevel].width), ky = kernelSize(img.lv[inLevel].height);
+  if(kx == 3 && ky == 3)
+    generalTwoGroup<F, CO, 3, 3>(tab, mid, img, inLevel, tilesX, tile);
+  else if(kx == 2 && ky == 2)
+    generalTwoGroup<F, CO, 2, 2>(tab, mid, img, inLevel, tilesX, tile);
+  else if(kx == 3 && ky == 2)
+    generalTwoGroup<F, CO, 3, 2>(tab, mid, img, inLevel, tilesX, tile);
+  else if(kx == 2 && ky == 3)
+    generalTwoGroup<F, CO, 2, 3>(tab, mid, img, inLevel, tilesX, tile);
+  else
+    generalTwoGroup<F, CO>(tab, mid, img, inLevel, tilesX, tile);
 }
 
 // ---------------------------------------------------------------------------
