@@ -47,7 +47,7 @@ fastPipelineKernel(ImageDesc img, uint32_t inLevel, uint32_t tilesX, uint32_t nu
         F::template fill<kGroupBS>(sh.tab, 1.0f);
         __syncthreads();
       }
-      tailRun<F, kGroupBS>(sh.tab, sh.level2, sh.level2 + kFastTailTexels, img, tail);
+      tailRun<F, kGroupBS, 1, false>(sh.tab, sh.level2, sh.level2 + kFastTailTexels, img, tail);
       return;
     }
     switch(next.count)

@@ -431,7 +431,7 @@ __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid,
     if(x < l2.width && y < l2.height)
     {
       auto at = [&](int j, int r) { return mid[(2u * oy + r) * G::MW + 2u * ox + j]; };
-      const V v = generalSample<F>(at, kx2, ky2, axis3(l2.width), axis3(l2.height), x, y);
+      const V v = generalSampleAny<F>(at, kx2, ky2, axis3(l2.width), axis3(l2.height), x, y);
       storeTexelTo<F, CO>(texelPtr(img, inLevel + 2u, x, y, F::kBytes), F::store(tab, v));
     }
   }
@@ -508,7 +508,9 @@ __device__ __forceinline__ void tailLoad(const ImageDesc& img, const LevelDesc& 
 // The first level reads memory: its outputs go in chunks of PF passes of the
 // workgroup, every load of a chunk issued before its arithmetic, so a chunk
 // pays one memory latency instead of one per pass (9 PF texels of registers).
-template <class F, int BS, int PF = 1, class TT>
+// SPEC: general levels through generalSampleAny (constant kernel sizes, more
+// code); off where registers are tight.
+template <class F, int BS, int PF = 1, bool SPEC = true, class TT>
 __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, typename F::Val* buf1,
                                         const ImageDesc& img, const TailPlan& plan)
 {
@@ -557,7 +559,8 @@ __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, ty
               const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
               auto at = [&](int j, int k) -> V { return F::template load<false>(tab, win[p].raw[k][j]); };
               finish(i, dx, dy, fast ? box<F, false>(at(0, 0), at(1, 0), at(0, 1), at(1, 1), order)
-                                     : generalSample<F>(at, kx, ky, ax, ay, dx, dy));
+                                     : (SPEC ? generalSampleAny<F>(at, kx, ky, ax, ay, dx, dy)
+                                            : generalSample<F>(at, kx, ky, ax, ay, dx, dy)));
             }
           }
         }
@@ -569,7 +572,8 @@ __device__ __forceinline__ void tailRun(const TT& tab, typename F::Val* buf0, ty
           const uint32_t dy = i / dl.width, dx = i - dy * dl.width;
           auto at = [&](int j, int k) -> V { return cur[(2u * dy + (uint32_t)k) * sl.width + 2u * dx + (uint32_t)j]; };
           finish(i, dx, dy, fast ? box<F, false>(at(0, 0), at(1, 0), at(0, 1), at(1, 1), order)
-                                 : generalSample<F>(at, kx, ky, ax, ay, dx, dy));
+                                 : (SPEC ? generalSampleAny<F>(at, kx, ky, ax, ay, dx, dy)
+                                            : generalSample<F>(at, kx, ky, ax, ay, dx, dy)));
         }
       }
       __syncthreads();

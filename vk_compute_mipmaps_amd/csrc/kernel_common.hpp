@@ -289,6 +289,25 @@ __device__ __forceinline__ typename F::Val generalSample(At at, int kx, int ky, 
   return horizontalTap<F>(ax, dx, kx, h[0], h[1], h[2]);
 }
 
+// generalSample with the kernel sizes as constants: kx and ky are the same for
+// every texel of a level, so this is a uniform branch to straight-line code
+// (no selects between the 2- and 3-tap forms).  Sides of 1 take the general
+// code.
+template <class F, class At>
+__device__ __forceinline__ typename F::Val generalSampleAny(At at, int kx, int ky, const Axis3& ax, const Axis3& ay,
+                                                            uint32_t dx, uint32_t dy)
+{
+  if(kx == 3 && ky == 3)
+    return generalSample<F>(at, 3, 3, ax, ay, dx, dy);
+  if(kx == 2 && ky == 2)
+    return generalSample<F>(at, 2, 2, ax, ay, dx, dy);
+  if(kx == 3 && ky == 2)
+    return generalSample<F>(at, 3, 2, ax, ay, dx, dy);
+  if(kx == 2 && ky == 3)
+    return generalSample<F>(at, 2, 3, ax, ay, dx, dy);
+  return generalSample<F>(at, kx, ky, ax, ay, dx, dy);
+}
+
 // Cross-lane moves of any value made of 32-bit words.
 // DPP row shift-left: lane i reads lane i + N of its 16-lane row (bound_ctrl: 0
 // beyond the row).  Only the lowest lane of each reduced group is consumed.
