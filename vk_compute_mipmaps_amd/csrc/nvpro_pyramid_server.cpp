@@ -17,6 +17,7 @@
 // arrive while others are on the device are admitted and overlap with them.
 // Every chain is planned exactly as nvproPyramidDispatch plans it, so a result
 // does not depend on what it was batched with.
+#include <stddef.h>
 #include <string.h>
 
 #include <chrono>
@@ -308,6 +309,53 @@ static int launch(NvproPyramidServer* s)
   return 0;
 }
 
+// Before the first request: one small synthetic chain per format through every
+// slot's generator and batch path, then a synchronize.  The first launch of a
+// kernel in a process pays for loading its code (milliseconds); without this
+// the first few hundred requests of a stream of arrivals would.  Uses the
+// (still empty) arena's first bytes.
+static hipError_t warmUp(NvproPyramidServer* s)
+{
+  const char* who = "nvproPyramidServerCreate";
+  for(Slot* slot : s->slots)
+  {
+    for(uint32_t format : {NVPRO_PYRAMID_FORMAT_SRGBA8, NVPRO_PYRAMID_FORMAT_RGBA16F})
+    {
+      const uint32_t w = 67, h = 35, bpt = nvproPyramidBytesPerTexel(format);  // odd: general steps and a tail
+      uint64_t bytes   = 0;
+      if(nvproPyramidPackedLayout(w, h, 0, bpt, nullptr, &bytes) != 0 || bytes > s->cfg.budgetBytes)
+        continue;  // an arena too small to hold it: the first request loads the kernels instead
+      const NvproPyramidImage img{s->arenaBase, w, h, 0, 0, nullptr, nullptr};
+      struct
+      {
+        SynthJob job;
+        SynthChunk chunk;
+      } table{SynthJob{(uint8_t*)s->arenaBase, 1u, w, h, w * bpt, format}, SynthChunk{0u, 0u, 0u, (uint64_t)w * h}};
+      void* dev     = nullptr;
+      hipError_t e  = slot->synth.upload(&table, sizeof table, slot->stream, &dev);
+      if(e == hipSuccess)
+        e = launchSynthBatch(slot->stream, (const SynthJob*)dev,
+                             (const SynthChunk*)((uint8_t*)dev + offsetof(decltype(table), chunk)), 1u);
+      const NvproPyramidPipelines pipes{
+          format, 1u, format == NVPRO_PYRAMID_FORMAT_RGBA16F ? s->cfg.arithmetic16f : NVPRO_PYRAMID_ARITH_FP32, 0u};
+      const uint32_t mips = 0;
+      BatchPlan plan;
+      if(e == hipSuccess)
+        if(int pe = buildBatchPlan(&pipes, &img, &mips, 1u, s->cfg.batch, plan, who))
+          return (hipError_t)pe;
+      if(e == hipSuccess)
+        e = slot->runner.stage(plan, pipes, s->cfg.batch, slot->stream);
+      if(e == hipSuccess)
+        e = slot->runner.launch(slot->stream);
+      if(e == hipSuccess)
+        e = hipStreamSynchronize(slot->stream);
+      if(e != hipSuccess)
+        return e;
+    }
+  }
+  return hipSuccess;
+}
+
 static void destroyServer(NvproPyramidServer* s)
 {
   for(Slot* slot : s->slots)
@@ -379,6 +427,8 @@ int nvproPyramidServerCreate(const NvproPyramidServerConfig* config, NvproPyrami
     if(e == hipSuccess)
       *slot->errHost = 0;
   }
+  if(e == hipSuccess)
+    e = warmUp(s);
   if(e != hipSuccess)
   {
     destroyServer(s);
