@@ -9,6 +9,7 @@ of general steps launch as before.  NVPRO_PYRAMID_FUSE=0 turns fusion off.  Both
 on the same stream (the arrival counters are left zeroed for the next launch)
 and on a second stream (each stream has its own counters).
 """
+import ctypes
 import os
 
 import numpy as np
@@ -145,3 +146,44 @@ def test_fused_concurrent_streams_srgba8(stream):
         hip_rt.stream_destroy(other)
         for c in chains:
             c.free()
+
+
+@pytest.mark.parametrize("w,h", [(2048, 2048), (128, 128), (4096, 64)])
+def test_fused_custom_layout_srgba8(w, h, stream):
+    """Fused launches with levels at arbitrary offsets and padded rows: the same
+    texels as the packed layout, padding untouched."""
+    l0 = H.level0_srgba8(w, h, 33)
+    packed = H.gpu_run(w, h, l0, stream)
+    levels = P.full_level_count(w, h)
+    offsets, pitches, off = [], [], 4096
+    for lvl in range(levels):
+        lw, lh = P.level_dims(w, h, lvl)
+        pitch = (lw * 4 + 255) // 256 * 256 + 4 * (lvl % 3)
+        offsets.append(off)
+        pitches.append(pitch)
+        off += pitch * lh + 1024
+    ptr = hip_rt.malloc(off)
+    try:
+        hip_rt.memset_d32_async(ptr, 0, off // 4, stream)
+        raw = np.ascontiguousarray(l0)
+        buf = np.zeros(pitches[0] * h, np.uint8)
+        for y in range(h):
+            buf[y * pitches[0]: y * pitches[0] + w * 4] = raw[y].reshape(-1)
+        hip_rt.stream_sync(stream)
+        hip_rt.memcpy_htod(ptr + offsets[0], buf.ctypes.data_as(ctypes.c_void_p), buf.nbytes)
+        img = P.make_image(ptr, w, h, levels, offsets, pitches)
+        P.nvpro_cmd_pyramid_dispatch(stream, P.pipelines(), img, 0)
+        launches = P.last_launch_count()
+        hip_rt.stream_sync(stream)
+        out = np.empty(off, np.uint8)
+        hip_rt.memcpy_dtoh(out.ctypes.data_as(ctypes.c_void_p), ptr, off)
+    finally:
+        hip_rt.free(ptr)
+    assert launches < len(P.plan(w, h)), "expected a fused dispatch"
+    for lvl, (o, lw, lh) in enumerate(H.level_slices(w, h)):
+        got = np.stack([out[offsets[lvl] + y * pitches[lvl]: offsets[lvl] + y * pitches[lvl] + lw * 4]
+                        for y in range(lh)])
+        np.testing.assert_array_equal(got.reshape(-1), packed[o:o + lw * lh * 4], err_msg=f"level {lvl}")
+        for y in range(lh):
+            pad = out[offsets[lvl] + y * pitches[lvl] + lw * 4: offsets[lvl] + (y + 1) * pitches[lvl]]
+            assert not pad.any(), f"level {lvl} row {y} padding written"
