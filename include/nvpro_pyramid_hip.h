@@ -144,11 +144,12 @@ uint32_t nvproPyramidDefaultGeneralDispatcher(NvproPyramidCmd* cmd, const NvproP
  * workgroups) whose successor is a fast step of one workgroup, or the rest of
  * the plan with at most 128 texels in its first output level, runs that in its
  * own launch, in whichever workgroup finishes last.  Results are the same bytes
- * either way.  This fusion needs 16.6 KB of arrival counters per stream,
- * allocated (hipMalloc, hipMemsetAsync on the stream) by the first such
- * dispatch on that stream and kept until the process ends; it is off while the
- * stream is being captured into a graph, and when the environment variable
- * NVPRO_PYRAMID_FUSE is "0". */
+ * either way.  This fusion needs a 16.6 KB block of arrival counters per
+ * stream, taken from a static device array of 64 blocks per device (nothing is
+ * allocated; the first 64 streams of a device that fuse keep a block until the
+ * process ends, later streams do not fuse); it is off while the stream is being
+ * captured into a graph, and when the environment variable NVPRO_PYRAMID_FUSE
+ * is "0". */
 int nvproPyramidDispatch(hipStream_t stream, const NvproPyramidPipelines* pipelines,
                          const NvproPyramidImage* image, uint32_t mipLevels);
 

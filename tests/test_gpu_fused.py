@@ -5,9 +5,11 @@ step's successor inside the step's own launch, in the workgroup that finishes
 last: a fast step of one group (the last step of 4096^2 and 2048^2) or a tail
 whose first level has at most 128 texels (8192^2 after F(6,6), 128^2), when
 the step's grid is small (at most 64 workgroups) or large (at least 1024).  Plans
-of general steps launch as before.  NVPRO_PYRAMID_FUSE=0 turns fusion off.  Both must agree byte for byte, also when the chain is dispatched again
-on the same stream (the arrival counters are left zeroed for the next launch)
-and on a second stream (each stream has its own counters).
+of general steps launch as before.  NVPRO_PYRAMID_FUSE=0 turns fusion off.
+Both must agree byte for byte, also when the chain is dispatched again on the
+same stream (the arrival counters are left zeroed for the next launch), on a
+second stream (each stream has its own counters) and on more streams than
+there are counter blocks (64 per device; later streams do not fuse).
 """
 import ctypes
 import os
@@ -187,3 +189,32 @@ def test_fused_custom_layout_srgba8(w, h, stream):
         for y in range(lh):
             pad = out[offsets[lvl] + y * pitches[lvl] + lw * 4: offsets[lvl] + (y + 1) * pitches[lvl]]
             assert not pad.any(), f"level {lvl} row {y} padding written"
+
+
+def test_fused_more_streams_than_counter_blocks_srgba8(stream):
+    """80 streams, each dispatching 4096^2 twice: the streams that get a counter
+    block fuse (1 launch), the rest do not (2 launches), and every result is the
+    separate-launch bytes."""
+    chain = P.DeviceMipChain(4096, 4096, P.FORMAT_SRGBA8)
+    streams = []
+    try:
+        chain.synthesize(9, stream)
+        want, _ = _run(chain, stream, False)
+        counts = []
+        for k in range(80):
+            s = hip_rt.stream_create()
+            streams.append(s)
+            for rep in range(2):
+                chain.poison(stream=s)
+                chain.dispatch(stream=s)
+                counts.append(P.last_launch_count())
+                hip_rt.stream_sync(s)
+                assert np.array_equal(chain.download(), want), f"stream {k} dispatch {rep} differs"
+        assert set(counts) <= {1, 2}
+        assert counts[-1] == 2, "a stream beyond the counter blocks fused"
+        assert counts.count(1) <= 2 * 64
+    finally:
+        for s in streams:
+            hip_rt.stream_sync(s)
+            hip_rt.stream_destroy(s)
+        chain.free()

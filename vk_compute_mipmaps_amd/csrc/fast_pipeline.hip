@@ -7,9 +7,43 @@
 // waves per SIMD, so the hardware overlaps one wave's loads with another's
 // arithmetic (four for 16-byte texels, whose 4x4 block alone takes 64
 // registers).
+#include <map>
+#include <mutex>
+
 #include "groups.hpp"
 
 namespace nvpro_pyramid {
+
+// The arrival counters of fused launches: zero when the code object is loaded,
+// and every fused launch leaves its block zeroed, so no memset is ever needed.
+// Per device, this array is the only device memory the dispatch path uses
+// besides the caller's image.
+static __device__ __align__(256) uint32_t gFusedCounters[kFusedSlots * kArriveWords];
+
+uint32_t* fusedCounterBlock(uint32_t slot)
+{
+  static std::mutex lock;
+  static std::map<int, uint32_t*> base;  // by device
+  int device = 0;
+  if(slot >= kFusedSlots || hipGetDevice(&device) != hipSuccess)
+  {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> guard(lock);
+  auto it = base.find(device);
+  if(it == base.end())
+  {
+    uint32_t* p = nullptr;
+    if(hipGetSymbolAddress((void**)&p, HIP_SYMBOL(gFusedCounters)) != hipSuccess)
+    {
+      (void)hipGetLastError();
+      p = nullptr;
+    }
+    it = base.emplace(device, p).first;
+  }
+  return it->second ? it->second + (size_t)slot * kArriveWords : nullptr;
+}
 
 template <class F, int COUNT, bool CONT>
 struct FastShared

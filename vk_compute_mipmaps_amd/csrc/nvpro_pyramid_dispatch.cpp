@@ -41,21 +41,22 @@ int fail(int code, const char* fmt, ...)
   return code;
 }
 
-// Arrival counters of fused launches (executePlan), one block per stream.  A
-// stream runs its launches one after another, so a block is never in use by
-// two launches at once, and every fused launch leaves it zeroed.  Streams are
-// told apart by hipStreamGetId, which a later stream does not reuse.  There is
-// no block (no fusion) while the stream is being captured: a graph may be
+// Arrival counters of fused launches (executePlan): a block per stream out of
+// the kFusedSlots blocks of a static device array (fusedCounterBlock), so the
+// dispatch path allocates nothing.  A stream runs its launches one after
+// another, so a block is never in use by two launches at once, and every fused
+// launch leaves it zeroed.  Streams are told apart by hipStreamGetId, which a
+// later stream does not reuse; a slot stays with its stream.  There is no
+// block (no fusion) while the stream is being captured: a graph may be
 // launched on any stream, beside eager launches on the capturing one, and
 // would share their block; nor for a stream of another device than the
-// current one, nor beyond kMaxFusedStreams streams, nor when
-// NVPRO_PYRAMID_FUSE=0 is set in the environment.
-constexpr size_t kMaxFusedStreams = 1024;
-
+// current one, nor once the slots are taken, nor when NVPRO_PYRAMID_FUSE=0 is
+// set in the environment.
 uint32_t* fusedCounters(hipStream_t stream)
 {
   static std::mutex lock;
-  static std::map<std::pair<int, unsigned long long>, uint32_t*> blocks;  // by current device, stream id
+  static std::map<std::pair<int, unsigned long long>, uint32_t> slots;  // by current device, stream id
+  static std::map<int, uint32_t> used;                                    // slots taken, by device
   const char* env = getenv("NVPRO_PYRAMID_FUSE");
   if(env && env[0] == '0')
     return nullptr;
@@ -70,31 +71,21 @@ uint32_t* fusedCounters(hipStream_t stream)
   }
   std::lock_guard<std::mutex> guard(lock);
   const auto key = std::make_pair(device, id);
-  const auto it  = blocks.find(key);
-  if(it != blocks.end())
-    return it->second;
+  const auto it  = slots.find(key);
+  if(it != slots.end())
+    return fusedCounterBlock(it->second);
   int streamDevice = 0;
-  if(blocks.size() >= kMaxFusedStreams || hipStreamGetDevice(stream, &streamDevice) != hipSuccess ||
+  if(used[device] >= kFusedSlots || hipStreamGetDevice(stream, &streamDevice) != hipSuccess ||
      device != streamDevice)
   {
     (void)hipGetLastError();
     return nullptr;
   }
-  void* p            = nullptr;
-  const size_t bytes = sizeof(uint32_t) * kArriveWords;
-  if(hipMalloc(&p, bytes) != hipSuccess)
-  {
-    (void)hipGetLastError();
+  uint32_t* block = fusedCounterBlock(used[device]);
+  if(!block)
     return nullptr;
-  }
-  if(hipMemsetAsync(p, 0, bytes, stream) != hipSuccess)
-  {
-    (void)hipGetLastError();
-    (void)hipFree(p);
-    return nullptr;
-  }
-  blocks[key] = static_cast<uint32_t*>(p);
-  return static_cast<uint32_t*>(p);
+  slots[key] = used[device]++;
+  return block;
 }
 
 }  // namespace

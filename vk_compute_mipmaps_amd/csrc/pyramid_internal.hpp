@@ -97,6 +97,13 @@ __host__ __device__ inline uint32_t batchCounterWords(uint32_t numSlots)
   return batchExitCounters(numSlots) + kArriveWords;
 }
 
+// Arrival counter blocks of fused launches: kFusedSlots blocks of kArriveWords
+// in a zero-initialised device array (fast_pipeline.hip), one per stream that
+// fuses, assigned by the dispatcher.  The block of `slot` on the current
+// device, or null (the symbol's address is unavailable).
+constexpr uint32_t kFusedSlots = 64;
+uint32_t* fusedCounterBlock(uint32_t slot);
+
 // What a fused launch runs in its last workgroup after its own step: a fast
 // step of one group (kNextFast: inLevel, count, tilesX, numTiles) or the tail
 // plan passed beside it (kNextTail).  arrive: kArriveWords zeroed counters that
