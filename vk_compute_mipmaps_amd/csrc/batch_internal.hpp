@@ -42,6 +42,9 @@ public:
                    hipStream_t stream);
   hipError_t launch(hipStream_t stream) const;
   uint32_t* errorWord() const;
+  // Grows the buffers to at least `bytes` now, so that work lists up to that
+  // size never allocate (hipFree of an outgrown buffer waits for the device).
+  hipError_t reserve(size_t bytes);
 
   uint32_t launches = 0, blocks = 0;
   size_t devBytes   = 0;
@@ -67,6 +70,7 @@ public:
   DeviceStaging& operator=(const DeviceStaging&) = delete;
   ~DeviceStaging();
   hipError_t upload(const void* host, size_t bytes, hipStream_t stream, void** device);
+  hipError_t reserve(size_t bytes);  // as BatchRunner::reserve
 
 private:
   void* dev_ = nullptr;

@@ -194,6 +194,32 @@ hipError_t launchBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic,
   }
 }
 
+// The host buffer is pinned memory the host wrote before the launch.  It is
+// read with system-scope loads, which do not hit lines a device cache may
+// still hold from an earlier batch's copy of the same buffer.
+__global__ void __launch_bounds__(256) stageCopyKernel(uint2* __restrict__ dst, const uint64_t* __restrict__ src,
+                                                       uint32_t copyWords, uint32_t totalWords)
+{
+  for(uint32_t i = blockIdx.x * 256u + threadIdx.x; i < totalWords; i += gridDim.x * 256u)
+  {
+    const uint64_t v = i < copyWords ? __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0u;
+    dst[i]           = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+  }
+}
+
+hipError_t launchStageCopy(hipStream_t stream, void* dst, const void* src, size_t copyBytes, size_t totalBytes)
+{
+  if(copyBytes > totalBytes || (copyBytes | totalBytes) % 16u != 0 || totalBytes / 8u > 0xffffffffull)
+    return hipErrorInvalidValue;
+  const uint32_t total = (uint32_t)(totalBytes / 8u);
+  if(total == 0)
+    return hipSuccess;
+  const uint32_t blocks = std::min<uint32_t>((total + 255u) / 256u, 64u);
+  hipLaunchKernelGGL(stageCopyKernel, dim3(blocks), dim3(256), 0, stream, (uint2*)dst, (const uint64_t*)src,
+                     (uint32_t)(copyBytes / 8u), total);
+  return hipGetLastError();
+}
+
 template <class F>
 static uint32_t maxWorkgroupsTyped(bool queue)
 {

@@ -208,30 +208,11 @@ hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& p
   // nothing in flight).  A serving loop stages a new work list per batch;
   // growing to the exact size would reallocate often, and hipFree waits for
   // the whole device.
-  if(size > devCap_)
-  {
-    const size_t want = std::max(size, 2u * devCap_);
-    if(dev_)
-      (void)hipFree(dev_);
-    dev_    = nullptr;
-    devCap_ = 0;
-    if(hipError_t e = hipMalloc(&dev_, want))
+  if(size > devCap_ || size > pinnedCap_)
+    if(hipError_t e = reserve(std::max(size, 2u * std::min(devCap_, pinnedCap_))))
       return e;
-    devCap_ = want;
-  }
-  if(size > pinnedCap_)
-  {
-    const size_t want = std::max(size, 2u * pinnedCap_);
-    if(pinned_)
-      (void)hipHostFree(pinned_);
-    pinned_    = nullptr;
-    pinnedCap_ = 0;
-    if(hipError_t e = hipHostMalloc(&pinned_, want, hipHostMallocDefault))
-      return e;
-    pinnedCap_ = want;
-  }
   uint8_t* h = (uint8_t*)pinned_;
-  memset(h, 0, size);
+  memset(h, 0, oCnt);
   memcpy(h + oImg, p.imgs.data(), p.imgs.size() * sizeof(ImageDesc));
   if(!p.segs.empty())
     memcpy(h + oSeg, p.segs.data(), p.segs.size() * sizeof(QueueSeg));
@@ -239,7 +220,8 @@ hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& p
     memcpy(h + oItm, p.items.data(), p.items.size() * sizeof(QueueItem));
   if(!p.tails.empty())
     memcpy(h + oTl, p.tails.data(), p.tails.size() * sizeof(TailPlan));
-  if(hipError_t e = hipMemcpyAsync(dev_, pinned_, size, hipMemcpyHostToDevice, stream))
+  // The work list is copied, the counters after it are zeroed on the device.
+  if(hipError_t e = launchStageCopy(stream, dev_, pinned_, oCnt, size))
     return e;
   uint8_t* base   = (uint8_t*)dev_;
   args_.imgs      = (const ImageDesc*)(base + oImg);
@@ -306,26 +288,60 @@ DeviceStaging::~DeviceStaging()
     (void)hipHostFree(pinned_);
 }
 
+hipError_t BatchRunner::reserve(size_t bytes)
+{
+  bytes = align256(bytes);
+  if(bytes > devCap_)
+  {
+    if(dev_)
+      (void)hipFree(dev_);
+    dev_    = nullptr;
+    devCap_ = 0;
+    if(hipError_t e = hipMalloc(&dev_, bytes))
+      return e;
+    devCap_ = bytes;
+  }
+  if(bytes > pinnedCap_)
+  {
+    if(pinned_)
+      (void)hipHostFree(pinned_);
+    pinned_    = nullptr;
+    pinnedCap_ = 0;
+    if(hipError_t e = hipHostMalloc(&pinned_, bytes, hipHostMallocDefault))
+      return e;
+    pinnedCap_ = bytes;
+  }
+  return hipSuccess;
+}
+
+hipError_t DeviceStaging::reserve(size_t bytes)
+{
+  bytes = align256(bytes);
+  if(bytes <= cap_)
+    return hipSuccess;
+  if(dev_)
+    (void)hipFree(dev_);
+  if(pinned_)
+    (void)hipHostFree(pinned_);
+  dev_ = pinned_ = nullptr;
+  cap_           = 0;
+  if(hipError_t e = hipMalloc(&dev_, bytes))
+    return e;
+  if(hipError_t e = hipHostMalloc(&pinned_, bytes, hipHostMallocDefault))
+    return e;
+  cap_ = bytes;
+  return hipSuccess;
+}
+
 hipError_t DeviceStaging::upload(const void* host, size_t bytes, hipStream_t stream, void** device)
 {
   if(bytes > cap_)
-  {
-    const size_t want = std::max(align256(bytes), 2u * cap_);  // geometric, as BatchRunner::stage
-    if(dev_)
-      (void)hipFree(dev_);
-    if(pinned_)
-      (void)hipHostFree(pinned_);
-    dev_ = pinned_ = nullptr;
-    cap_           = 0;
-    if(hipError_t e = hipMalloc(&dev_, want))
+    if(hipError_t e = reserve(std::max(bytes, 2u * cap_)))  // geometric, as BatchRunner::stage
       return e;
-    if(hipError_t e = hipHostMalloc(&pinned_, want, hipHostMallocDefault))
-      return e;
-    cap_ = want;
-  }
   memcpy(pinned_, host, bytes);
-  *device = dev_;
-  return hipMemcpyAsync(dev_, pinned_, bytes, hipMemcpyHostToDevice, stream);
+  *device             = dev_;
+  const size_t padded = (bytes + 15u) & ~size_t(15u);  // within cap_ (a multiple of 256)
+  return launchStageCopy(stream, dev_, pinned_, padded, padded);
 }
 
 uint32_t* BatchRunner::errorWord() const { return args_.counters ? args_.counters + kCounterError : nullptr; }

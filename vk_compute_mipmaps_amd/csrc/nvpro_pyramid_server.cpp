@@ -20,6 +20,7 @@
 #include <stddef.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <deque>
 #include <map>
@@ -415,6 +416,12 @@ int nvproPyramidServerCreate(const NvproPyramidServerConfig* config, NvproPyrami
   NvproPyramidServer* s = new NvproPyramidServer;
   s->cfg                = cfg;
   hipError_t e          = hipMalloc(&s->arenaBase, cfg.budgetBytes);
+  // Work lists and level-0 generator tables take about 1.5 KB per image of a
+  // mixed-size batch; sized now for a full batch of such images, so that the
+  // staging of ordinary batches never grows (growth frees, and hipFree waits
+  // for every batch in flight).
+  const size_t stageReserve =
+      std::min<size_t>(8u << 20, std::max<size_t>(256u << 10, (size_t)cfg.maxBatchImages * 2048u));
   for(uint32_t i = 0; e == hipSuccess && i < cfg.maxInFlight; ++i)
   {
     Slot* slot = new Slot;
@@ -426,6 +433,10 @@ int nvproPyramidServerCreate(const NvproPyramidServerConfig* config, NvproPyrami
       e = hipHostMalloc((void**)&slot->errHost, sizeof(uint32_t), hipHostMallocDefault);
     if(e == hipSuccess)
       *slot->errHost = 0;
+    if(e == hipSuccess)
+      e = slot->runner.reserve(stageReserve);
+    if(e == hipSuccess)
+      e = slot->synth.reserve(stageReserve);
   }
   if(e == hipSuccess)
     e = warmUp(s);

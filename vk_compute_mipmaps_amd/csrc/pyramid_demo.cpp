@@ -13,7 +13,7 @@
 //       Times a loop of plain per-texture dispatches (eager, and captured into
 //       one hipGraph like a recorded command buffer) against one batched launch,
 //       checks that both write identical bytes, prints one JSON object.
-//   nvpro_pyramid_demo serve [--rate R] [--count N] [--seed S] [--hold-ms H]
+//   nvpro_pyramid_demo serve [--rate R] [--count N] [--seed S] [--hold-ms H] [--latency-csv PATH]
 //                            [--budget-gb B] [--in-flight K] [--max-batch M] [--packed]
 //                            [--queue | --rounds]
 //       Serving loop: N requests arrive as a Poisson stream of R per second;
@@ -21,7 +21,8 @@
 //       is the seeded synthetic image, written on the device at launch.  The
 //       server admits against a B GB budget and batches what arrived; each
 //       completed chain stays resident for H ms before the client releases it.
-//       Prints chains/s, p50/p99 latency and batching statistics as JSON.
+//       Prints chains/s, p50/p99 latency and batching statistics as JSON;
+//       --latency-csv also writes every request's arrival, size and latency.
 //   nvpro_pyramid_demo sizes N S
 //       Prints the workload's N texture sizes for seed S (no device use).
 #include <hip/hip_runtime.h>
@@ -33,6 +34,7 @@
 #include <algorithm>
 #include <chrono>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "nvpro_pyramid_hip.h"
@@ -299,6 +301,7 @@ struct ServeArgs
   uint64_t seed = 9;
   bool packed = false;
   int mode    = -1;  // batch mode; -1: the server's default
+  const char* latencyCsv = nullptr;  // per request: index, arrival s, width, height, format, latency us
 };
 
 static int serve(const ServeArgs& a)
@@ -330,10 +333,14 @@ static int serve(const ServeArgs& a)
   std::vector<NvproPyramidCompletion> done(4096);
   std::vector<double> latency, wait;
   std::vector<std::pair<double, uint64_t>> holding;  // (release time, ticket)
+  std::unordered_map<uint64_t, uint32_t> indexOf;      // ticket -> request index (latencyCsv)
+  std::vector<std::pair<uint32_t, double>> order;      // (request index, latency us)
   uint64_t bytes = 0;
   uint32_t next = 0, finished = 0, failed = 0;
   const Clock::time_point t0 = Clock::now();
   double lastDone            = 0;
+  double lastIter = 0, gapMax = 0, pollMax = 0;  // host loop: longest gap between iterations, longest poll
+  uint32_t slowPolls = 0;                        // polls over 1 ms
   while(finished < a.count)
   {
     const double now = std::chrono::duration<double>(Clock::now() - t0).count();
@@ -343,13 +350,24 @@ static int serve(const ServeArgs& a)
                             nullptr};
       uint64_t ticket;
       CHECK_PYR(nvproPyramidServerSubmit(srv, &r, &ticket));
+      if(a.latencyCsv)
+        indexOf[ticket] = next;
       ++next;
     }
     uint32_t n = 0;
+    const Clock::time_point p0 = Clock::now();
     CHECK_PYR(nvproPyramidServerPoll(srv, done.data(), (uint32_t)done.size(), &n));
+    const double pollUs = std::chrono::duration<double, std::micro>(Clock::now() - p0).count();
+    pollMax             = std::max(pollMax, pollUs);
+    slowPolls += pollUs > 1000.0 ? 1u : 0u;
+    const double gapUs = (now - lastIter) * 1e6;
+    gapMax             = std::max(gapMax, gapUs);
+    lastIter           = now;
     for(uint32_t i = 0; i < n; ++i)
     {
       latency.push_back(done[i].latencyUs);
+      if(a.latencyCsv)
+        order.push_back({indexOf[done[i].ticket], done[i].latencyUs});
       wait.push_back(done[i].waitUs);
       bytes += done[i].bytes;
       failed += done[i].status != 0 ? 1u : 0u;
@@ -373,6 +391,18 @@ static int serve(const ServeArgs& a)
     CHECK_PYR(nvproPyramidServerRelease(srv, h.second));
   CHECK_PYR(nvproPyramidServerDestroy(srv));
   const double span = lastDone;
+  if(a.latencyCsv)
+  {
+    FILE* f = fopen(a.latencyCsv, "w");
+    if(!f)
+      return 4;
+    fprintf(f, "index,arrival_s,width,height,format,latency_us\n");
+    std::sort(order.begin(), order.end());
+    for(const auto& o : order)
+      fprintf(f, "%u,%.6f,%u,%u,%u,%.1f\n", o.first, arrival[o.first], sizes[o.first].first, sizes[o.first].second,
+              format[o.first], o.second);
+    fclose(f);
+  }
   printf("{\"requests\": %u, \"rate_per_s\": %.1f, \"mix\": \"70%% sRGBA8 / 30%% RGBA16F (%s)\", "
          "\"budget_bytes\": %llu, \"in_flight\": %u, \"hold_ms\": %.1f, \"batch_mode\": \"%s\",\n",
          a.count, a.rate, a.packed ? "packed_fp16" : "fp32", (unsigned long long)cfg.budgetBytes, a.inFlight,
@@ -383,7 +413,13 @@ static int serve(const ServeArgs& a)
          "\"admission_wait_us\": {\"p50\": %.1f, \"p99\": %.1f},\n",
          percentile(latency, 0.5), percentile(latency, 0.9), percentile(latency, 0.99), percentile(latency, 1.0),
          percentile(wait, 0.5), percentile(wait, 0.99));
-  printf(" \"batches\": %llu, \"mean_batch_images\": %.2f, \"max_batch_images\": %u, "
+  // The second half of the completions alone: steady state, without the start.
+  const std::vector<double> late(latency.begin() + latency.size() / 2, latency.end());
+  printf(" \"latency_us_second_half\": {\"p50\": %.1f, \"p99\": %.1f, \"max\": %.1f},\n", percentile(late, 0.5),
+         percentile(late, 0.99), percentile(late, 1.0));
+  printf(" \"host_loop_us\": {\"max_gap\": %.1f, \"max_poll\": %.1f, \"polls_over_1ms\": %u},\n", gapMax, pollMax,
+         slowPolls);
+  printf(" \"batches\": %llu, \"mean_batch_images\": %.2f, \"batch_image_cap\": %u, "
          "\"peak_resident_bytes\": %llu}\n",
          (unsigned long long)st.batches, st.batches ? (double)st.batchedImages / st.batches : 0.0,
          cfg.maxBatchImages, (unsigned long long)st.peakResidentBytes);
@@ -418,6 +454,8 @@ int main(int argc, char** argv)
         a.mode = NVPRO_PYRAMID_BATCH_MODE_QUEUE;
       else if(o == "--rounds")
         a.mode = NVPRO_PYRAMID_BATCH_MODE_ROUNDS;
+      else if(o == "--latency-csv" && i + 1 < argc)
+        a.latencyCsv = argv[++i];
       else
       {
         fprintf(stderr, "unknown option %s\n", o.c_str());

@@ -144,6 +144,13 @@ hipError_t launchBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic,
                        uint32_t blocks);
 // Resident workgroups per compute unit of the batched kernel.
 uint32_t batchMaxWorkgroupsPerCU(uint32_t format, uint32_t arithmetic, bool queue);
+// Copies copyBytes from pinned host memory `src` to device memory `dst` and
+// zero-fills dst up to totalBytes, in one kernel on `stream` (both sizes
+// multiples of 16, copyBytes <= totalBytes).  Used for work lists instead of
+// hipMemcpyAsync, whose host-to-device copies of a few tens of KB were
+// measured to block the calling thread for about 10 ms now and then while
+// other streams had work in flight.
+hipError_t launchStageCopy(hipStream_t stream, void* dst, const void* src, size_t copyBytes, size_t totalBytes);
 
 // Seeded synthetic level 0 (synth.hip): one image per job; a batched launch
 // writes chunks of kSynthChunk texels of many jobs.
