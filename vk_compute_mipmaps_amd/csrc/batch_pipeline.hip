@@ -220,6 +220,23 @@ hipError_t launchStageCopy(hipStream_t stream, void* dst, const void* src, size_
   return hipGetLastError();
 }
 
+// A system-scope store: written through to the host's memory, where the host
+// reads it once the stream's later event has completed.
+__global__ void __launch_bounds__(64) readWordKernel(uint32_t* hostDst, const uint32_t* devSrc)
+{
+  if(threadIdx.x == 0)
+    __hip_atomic_store(hostDst, __hip_atomic_load(devSrc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+hipError_t launchReadWord(hipStream_t stream, uint32_t* hostDst, const uint32_t* devSrc)
+{
+  if(!hostDst || !devSrc)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(readWordKernel, dim3(1), dim3(64), 0, stream, hostDst, devSrc);
+  return hipGetLastError();
+}
+
 template <class F>
 static uint32_t maxWorkgroupsTyped(bool queue)
 {

@@ -296,8 +296,7 @@ static int launch(NvproPyramidServer* s)
     if(e == hipSuccess)
       e = slot->runner.launch(slot->stream);
     if(e == hipSuccess)
-      e = hipMemcpyAsync(slot->errHost, slot->runner.errorWord(), sizeof(uint32_t), hipMemcpyDeviceToHost,
-                         slot->stream);
+      e = launchReadWord(slot->stream, slot->errHost, slot->runner.errorWord());
     if(e == hipSuccess)
       e = hipEventRecord(slot->done, slot->stream);
     if(e != hipSuccess)

@@ -151,6 +151,9 @@ uint32_t batchMaxWorkgroupsPerCU(uint32_t format, uint32_t arithmetic, bool queu
 // measured to block the calling thread for about 10 ms now and then while
 // other streams had work in flight.
 hipError_t launchStageCopy(hipStream_t stream, void* dst, const void* src, size_t copyBytes, size_t totalBytes);
+// Copies one device word to pinned host memory in a one-wave kernel on
+// `stream` (for a batch's error flags: cheaper than a device-to-host copy).
+hipError_t launchReadWord(hipStream_t stream, uint32_t* hostDst, const uint32_t* devSrc);
 
 // Seeded synthetic level 0 (synth.hip): one image per job; a batched launch
 // writes chunks of kSynthChunk texels of many jobs.
@@ -165,7 +168,7 @@ struct SynthChunk
   uint32_t job, pad;
   uint64_t first, count;  // texels [first, first + count), row-major
 };
-constexpr uint64_t kSynthChunk = 16384;
+constexpr uint64_t kSynthChunk = 4096;  // 16 texels per thread: enough workgroups for one 1 MP image
 hipError_t launchSynthBatch(hipStream_t stream, const SynthJob* jobs, const SynthChunk* chunks, uint32_t numChunks);
 
 // Launch a validated plan: per-step kernels for the large steps, then one tail

@@ -39,7 +39,7 @@ __device__ __forceinline__ uint32_t synthSrgba8Texel(const uint64_t (&sm)[8], ui
     const uint64_t top = L00 * (D - sx) + L10 * sx, bot = L01 * (D - sx) + L11 * sx;
     const uint64_t num = top * (D - sy) + bot * sy;
     const int64_t v    = (int64_t)((num + (1ull << 35)) >> 36);
-    const int64_t nz   = (int64_t)((hash4(sm[4 + c], x, y) >> 32) % 33u) - 16;
+    const int64_t nz   = (int64_t)((uint32_t)(hash4(sm[4 + c], x, y) >> 32) % 33u) - 16;  // 32-bit remainder
     int64_t q          = v + nz;
     q                  = q < 0 ? 0 : (q > 255 ? 255 : q);
     texel |= (uint32_t)q << (8 * c);
@@ -80,7 +80,7 @@ __device__ __forceinline__ void seedStreams(uint64_t seed, uint64_t (&sm)[8])
 // Lattice values of the sRGBA8 value noise a chunk needs, per channel: cell
 // columns 0 .. ni-1 and rows j0 .. j0+nj-1 (one byte each).  Every texel of a
 // 64x64 cell uses the same four corners, so a chunk computes them once.
-constexpr uint32_t kLatticeMax = 4096;  // per channel; a chunk needs at most 3075
+constexpr uint32_t kLatticeMax = 4096;  // per channel; a chunk needs at most 3075 (3 cell rows x 1025 columns)
 struct LatticeCache
 {
   uint8_t v[3][kLatticeMax];
@@ -102,7 +102,7 @@ __device__ __forceinline__ uint32_t synthSrgba8TexelCached(const uint64_t (&sm)[
     const uint64_t top = L00 * (D - sx) + L10 * sx, bot = L01 * (D - sx) + L11 * sx;
     const uint64_t num = top * (D - sy) + bot * sy;
     const int64_t v    = (int64_t)((num + (1ull << 35)) >> 36);
-    const int64_t nz   = (int64_t)((hash4(sm[4 + c], x, y) >> 32) % 33u) - 16;
+    const int64_t nz   = (int64_t)((uint32_t)(hash4(sm[4 + c], x, y) >> 32) % 33u) - 16;  // 32-bit remainder
     int64_t q          = v + nz;
     q                  = q < 0 ? 0 : (q > 255 ? 255 : q);
     texel |= (uint32_t)q << (8 * c);
