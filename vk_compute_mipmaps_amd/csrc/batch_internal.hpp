@@ -38,8 +38,13 @@ public:
   BatchRunner(const BatchRunner&) = delete;
   BatchRunner& operator=(const BatchRunner&) = delete;
   ~BatchRunner();
+  // `extra` (extraBytes, may be 0) is staged in the same copy, at extraDevice().
   hipError_t stage(const BatchPlan& plan, const NvproPyramidPipelines& pipes, const NvproPyramidBatchConfig& cfg,
-                   hipStream_t stream);
+                   hipStream_t stream, const void* extra = nullptr, size_t extraBytes = 0);
+  void* extraDevice() const { return extra_; }
+  // Queue mode: the launch's last workgroup stores the error flags at `host`
+  // (pinned, or null for none).
+  void setErrorHost(uint32_t* host) { args_.errorHost = host; }
   hipError_t launch(hipStream_t stream) const;
   uint32_t* errorWord() const;
   // Grows the buffers to at least `bytes` now, so that work lists up to that
@@ -52,30 +57,12 @@ public:
 private:
   void* dev_ = nullptr;
   void* pinned_ = nullptr;
+  void* extra_  = nullptr;
   size_t devCap_ = 0, pinnedCap_ = 0;
   BatchArgs args_{};
   uint32_t format_ = 0, arithmetic_ = 0, items_ = 0;
   bool queue_ = true;
   std::vector<uint32_t> roundEnd_;
-};
-
-// A grow-only device buffer with pinned staging: upload() copies `bytes` of
-// host data to the device on `stream`.  Like BatchRunner::stage, it must not be
-// called again before the previous copy and its users have finished.
-class DeviceStaging
-{
-public:
-  DeviceStaging() = default;
-  DeviceStaging(const DeviceStaging&) = delete;
-  DeviceStaging& operator=(const DeviceStaging&) = delete;
-  ~DeviceStaging();
-  hipError_t upload(const void* host, size_t bytes, hipStream_t stream, void** device);
-  hipError_t reserve(size_t bytes);  // as BatchRunner::reserve
-
-private:
-  void* dev_ = nullptr;
-  void* pinned_ = nullptr;
-  size_t cap_ = 0;
 };
 
 }  // namespace nvpro_pyramid

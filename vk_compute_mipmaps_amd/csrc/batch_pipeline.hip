@@ -50,8 +50,8 @@ struct BatchShared
 // the restrict-qualified pointers let its fields live in scalar registers.
 #define NVPRO_BATCH_PARAMS                                                                                          \
   const ImageDesc *__restrict__ imgs, const QueueSeg *__restrict__ segs, const QueueItem *__restrict__ items,       \
-      const TailPlan *__restrict__ tails, uint32_t *__restrict__ counters, uint32_t itemBegin, uint32_t itemEnd,    \
-      uint32_t numSlots, uint32_t spinLimit
+      const TailPlan *__restrict__ tails, uint32_t *__restrict__ counters, uint32_t *__restrict__ errorHost,        \
+      uint32_t itemBegin, uint32_t itemEnd, uint32_t numSlots, uint32_t spinLimit
 
 // CO: the step's last level is stored coherently, for the queue mode's
 // successors in the same launch.
@@ -159,7 +159,14 @@ batchQueueKernel(NVPRO_BATCH_PARAMS)
     for(uint32_t i = threadIdx.x; i < numSlots; i += kGroupBS)
       __hip_atomic_store(slots + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if(threadIdx.x == 0)
+    {
       __hip_atomic_store(head, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      // Every wait has ended by now; the flags go straight to the host (a
+      // system-scope store), saving a copy after the launch.
+      if(errorHost)
+        __hip_atomic_store(errorHost, __hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
   }
 }
 
@@ -168,10 +175,10 @@ static hipError_t launchBatchTyped(hipStream_t stream, bool queue, const BatchAr
 {
   if(queue)
     hipLaunchKernelGGL((batchQueueKernel<F>), dim3(blocks), dim3(kGroupBS), 0, stream, a.imgs, a.segs, a.items,
-                       a.tails, a.counters, a.itemBegin, a.itemEnd, a.numSlots, a.spinLimit);
+                       a.tails, a.counters, a.errorHost, a.itemBegin, a.itemEnd, a.numSlots, a.spinLimit);
   else
     hipLaunchKernelGGL((batchRoundKernel<F>), dim3(blocks), dim3(kGroupBS), 0, stream, a.imgs, a.segs, a.items,
-                       a.tails, a.counters, a.itemBegin, a.itemEnd, a.numSlots, a.spinLimit);
+                       a.tails, a.counters, a.errorHost, a.itemBegin, a.itemEnd, a.numSlots, a.spinLimit);
   return hipGetLastError();
 }
 
@@ -217,23 +224,6 @@ hipError_t launchStageCopy(hipStream_t stream, void* dst, const void* src, size_
   const uint32_t blocks = std::min<uint32_t>((total + 255u) / 256u, 64u);
   hipLaunchKernelGGL(stageCopyKernel, dim3(blocks), dim3(256), 0, stream, (uint2*)dst, (const uint64_t*)src,
                      (uint32_t)(copyBytes / 8u), total);
-  return hipGetLastError();
-}
-
-// A system-scope store: written through to the host's memory, where the host
-// reads it once the stream's later event has completed.
-__global__ void __launch_bounds__(64) readWordKernel(uint32_t* hostDst, const uint32_t* devSrc)
-{
-  if(threadIdx.x == 0)
-    __hip_atomic_store(hostDst, __hip_atomic_load(devSrc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-
-hipError_t launchReadWord(hipStream_t stream, uint32_t* hostDst, const uint32_t* devSrc)
-{
-  if(!hostDst || !devSrc)
-    return hipErrorInvalidValue;
-  hipLaunchKernelGGL(readWordKernel, dim3(1), dim3(64), 0, stream, hostDst, devSrc);
   return hipGetLastError();
 }
 

@@ -195,14 +195,16 @@ BatchRunner::~BatchRunner()
 }
 
 hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& pipes,
-                              const NvproPyramidBatchConfig& cfg, hipStream_t stream)
+                              const NvproPyramidBatchConfig& cfg, hipStream_t stream, const void* extra,
+                              size_t extraBytes)
 {
-  // Device layout: images | segments | items | tails | counters (zeroed).
+  // Device layout: images | segments | items | tails | extra | counters (zeroed).
   const size_t oImg = 0;
   const size_t oSeg = align256(oImg + p.imgs.size() * sizeof(ImageDesc));
   const size_t oItm = align256(oSeg + p.segs.size() * sizeof(QueueSeg));
   const size_t oTl  = align256(oItm + p.items.size() * sizeof(QueueItem));
-  const size_t oCnt = align256(oTl + p.tails.size() * sizeof(TailPlan));
+  const size_t oExt = align256(oTl + p.tails.size() * sizeof(TailPlan));
+  const size_t oCnt = align256(oExt + (extra ? extraBytes : 0u));
   const size_t size = align256(oCnt + batchCounterWords(p.numSlots) * sizeof(uint32_t));
   // Grow geometrically (the previous buffers are idle: the caller staged
   // nothing in flight).  A serving loop stages a new work list per batch;
@@ -220,6 +222,8 @@ hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& p
     memcpy(h + oItm, p.items.data(), p.items.size() * sizeof(QueueItem));
   if(!p.tails.empty())
     memcpy(h + oTl, p.tails.data(), p.tails.size() * sizeof(TailPlan));
+  if(extra && extraBytes)
+    memcpy(h + oExt, extra, extraBytes);
   // The work list is copied, the counters after it are zeroed on the device.
   if(hipError_t e = launchStageCopy(stream, dev_, pinned_, oCnt, size))
     return e;
@@ -229,6 +233,7 @@ hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& p
   args_.items     = (const QueueItem*)(base + oItm);
   args_.tails     = (const TailPlan*)(base + oTl);
   args_.counters  = (uint32_t*)(base + oCnt);
+  extra_          = extra ? base + oExt : nullptr;
   args_.numSlots  = p.numSlots;
   args_.spinLimit = kSpinLimit;
   format_         = pipes.format;
@@ -280,14 +285,6 @@ hipError_t BatchRunner::launch(hipStream_t stream) const
   return hipSuccess;
 }
 
-DeviceStaging::~DeviceStaging()
-{
-  if(dev_)
-    (void)hipFree(dev_);
-  if(pinned_)
-    (void)hipHostFree(pinned_);
-}
-
 hipError_t BatchRunner::reserve(size_t bytes)
 {
   bytes = align256(bytes);
@@ -312,36 +309,6 @@ hipError_t BatchRunner::reserve(size_t bytes)
     pinnedCap_ = bytes;
   }
   return hipSuccess;
-}
-
-hipError_t DeviceStaging::reserve(size_t bytes)
-{
-  bytes = align256(bytes);
-  if(bytes <= cap_)
-    return hipSuccess;
-  if(dev_)
-    (void)hipFree(dev_);
-  if(pinned_)
-    (void)hipHostFree(pinned_);
-  dev_ = pinned_ = nullptr;
-  cap_           = 0;
-  if(hipError_t e = hipMalloc(&dev_, bytes))
-    return e;
-  if(hipError_t e = hipHostMalloc(&pinned_, bytes, hipHostMallocDefault))
-    return e;
-  cap_ = bytes;
-  return hipSuccess;
-}
-
-hipError_t DeviceStaging::upload(const void* host, size_t bytes, hipStream_t stream, void** device)
-{
-  if(bytes > cap_)
-    if(hipError_t e = reserve(std::max(bytes, 2u * cap_)))  // geometric, as BatchRunner::stage
-      return e;
-  memcpy(pinned_, host, bytes);
-  *device             = dev_;
-  const size_t padded = (bytes + 15u) & ~size_t(15u);  // within cap_ (a multiple of 256)
-  return launchStageCopy(stream, dev_, pinned_, padded, padded);
 }
 
 uint32_t* BatchRunner::errorWord() const { return args_.counters ? args_.counters + kCounterError : nullptr; }

@@ -123,9 +123,8 @@ struct Slot
 {
   hipStream_t stream = nullptr;
   hipEvent_t done    = nullptr;
-  uint32_t* errHost  = nullptr;  // pinned copy of the batch's error flags
-  BatchRunner runner;
-  DeviceStaging synth;  // level-0 generator jobs and chunks of the batch
+  uint32_t* errHost  = nullptr;  // pinned copy of the batch's error flags (queue mode writes it)
+  BatchRunner runner;           // work list, and the level-0 generator's jobs and chunks
   std::vector<uint64_t> tickets;
   bool busy = false;
 };
@@ -270,20 +269,14 @@ static int launch(NvproPyramidServer* s)
       }
       r.state = State::Running;
     }
+    // The level-0 generator's table (jobs, then chunks) travels with the work
+    // list in one staging copy; one launch then writes every synthetic level 0
+    // of the batch.
+    std::vector<uint8_t> table(jobs.size() * sizeof(SynthJob) + chunks.size() * sizeof(SynthChunk));
     if(!chunks.empty())
     {
-      // One launch writes every synthetic level 0 of the batch: jobs, then chunks.
-      std::vector<uint8_t> table(jobs.size() * sizeof(SynthJob) + chunks.size() * sizeof(SynthChunk));
       memcpy(table.data(), jobs.data(), jobs.size() * sizeof(SynthJob));
       memcpy(table.data() + jobs.size() * sizeof(SynthJob), chunks.data(), chunks.size() * sizeof(SynthChunk));
-      void* dev        = nullptr;
-      hipError_t he    = slot->synth.upload(table.data(), table.size(), slot->stream, &dev);
-      if(he == hipSuccess)
-        he = launchSynthBatch(slot->stream, (const SynthJob*)dev,
-                              (const SynthChunk*)((uint8_t*)dev + jobs.size() * sizeof(SynthJob)),
-                              (uint32_t)chunks.size());
-      if(he != hipSuccess)
-        return hostFail(he, "%s: level 0 generation: %s", who, hipGetErrorString(he));
     }
     const NvproPyramidPipelines pipes{format, 1u,
                                       format == NVPRO_PYRAMID_FORMAT_RGBA16F ? s->cfg.arithmetic16f
@@ -292,11 +285,17 @@ static int launch(NvproPyramidServer* s)
     BatchPlan plan;
     if(int e = buildBatchPlan(&pipes, images.data(), mips.data(), (uint32_t)images.size(), s->cfg.batch, plan, who))
       return e;
-    hipError_t e = slot->runner.stage(plan, pipes, s->cfg.batch, slot->stream);
+    *slot->errHost = 0;  // the slot is idle: no launch writes it now
+    hipError_t e   = slot->runner.stage(plan, pipes, s->cfg.batch, slot->stream, chunks.empty() ? nullptr : table.data(),
+                                        table.size());
+    if(e == hipSuccess && !chunks.empty())
+    {
+      const uint8_t* dev = (const uint8_t*)slot->runner.extraDevice();
+      e = launchSynthBatch(slot->stream, (const SynthJob*)dev, (const SynthChunk*)(dev + jobs.size() * sizeof(SynthJob)),
+                           (uint32_t)chunks.size());
+    }
     if(e == hipSuccess)
       e = slot->runner.launch(slot->stream);
-    if(e == hipSuccess)
-      e = launchReadWord(slot->stream, slot->errHost, slot->runner.errorWord());
     if(e == hipSuccess)
       e = hipEventRecord(slot->done, slot->stream);
     if(e != hipSuccess)
@@ -331,20 +330,19 @@ static hipError_t warmUp(NvproPyramidServer* s)
         SynthJob job;
         SynthChunk chunk;
       } table{SynthJob{(uint8_t*)s->arenaBase, 1u, w, h, w * bpt, format}, SynthChunk{0u, 0u, 0u, (uint64_t)w * h}};
-      void* dev     = nullptr;
-      hipError_t e  = slot->synth.upload(&table, sizeof table, slot->stream, &dev);
-      if(e == hipSuccess)
-        e = launchSynthBatch(slot->stream, (const SynthJob*)dev,
-                             (const SynthChunk*)((uint8_t*)dev + offsetof(decltype(table), chunk)), 1u);
       const NvproPyramidPipelines pipes{
           format, 1u, format == NVPRO_PYRAMID_FORMAT_RGBA16F ? s->cfg.arithmetic16f : NVPRO_PYRAMID_ARITH_FP32, 0u};
       const uint32_t mips = 0;
       BatchPlan plan;
+      if(int pe = buildBatchPlan(&pipes, &img, &mips, 1u, s->cfg.batch, plan, who))
+        return (hipError_t)pe;
+      hipError_t e = slot->runner.stage(plan, pipes, s->cfg.batch, slot->stream, &table, sizeof table);
       if(e == hipSuccess)
-        if(int pe = buildBatchPlan(&pipes, &img, &mips, 1u, s->cfg.batch, plan, who))
-          return (hipError_t)pe;
-      if(e == hipSuccess)
-        e = slot->runner.stage(plan, pipes, s->cfg.batch, slot->stream);
+      {
+        const uint8_t* dev = (const uint8_t*)slot->runner.extraDevice();
+        e = launchSynthBatch(slot->stream, (const SynthJob*)dev,
+                             (const SynthChunk*)(dev + offsetof(decltype(table), chunk)), 1u);
+      }
       if(e == hipSuccess)
         e = slot->runner.launch(slot->stream);
       if(e == hipSuccess)
@@ -415,12 +413,12 @@ int nvproPyramidServerCreate(const NvproPyramidServerConfig* config, NvproPyrami
   NvproPyramidServer* s = new NvproPyramidServer;
   s->cfg                = cfg;
   hipError_t e          = hipMalloc(&s->arenaBase, cfg.budgetBytes);
-  // Work lists and level-0 generator tables take about 1.5 KB per image of a
-  // mixed-size batch; sized now for a full batch of such images, so that the
-  // staging of ordinary batches never grows (growth frees, and hipFree waits
-  // for every batch in flight).
+  // A work list with the level-0 generator's table takes about 3 KB per image
+  // of a mixed-size batch; sized now for a full batch of such images, so that
+  // the staging of ordinary batches never grows (growth frees, and hipFree
+  // waits for every batch in flight).
   const size_t stageReserve =
-      std::min<size_t>(8u << 20, std::max<size_t>(256u << 10, (size_t)cfg.maxBatchImages * 2048u));
+      std::min<size_t>(16u << 20, std::max<size_t>(256u << 10, (size_t)cfg.maxBatchImages * 4096u));
   for(uint32_t i = 0; e == hipSuccess && i < cfg.maxInFlight; ++i)
   {
     Slot* slot = new Slot;
@@ -434,8 +432,8 @@ int nvproPyramidServerCreate(const NvproPyramidServerConfig* config, NvproPyrami
       *slot->errHost = 0;
     if(e == hipSuccess)
       e = slot->runner.reserve(stageReserve);
-    if(e == hipSuccess)
-      e = slot->synth.reserve(stageReserve);
+    if(cfg.batch.mode == NVPRO_PYRAMID_BATCH_MODE_QUEUE)
+      slot->runner.setErrorHost(slot->errHost);  // rounds mode has no waits, so no error flags
   }
   if(e == hipSuccess)
     e = warmUp(s);

@@ -138,6 +138,7 @@ struct BatchArgs
   const QueueItem* items;
   const TailPlan* tails;
   uint32_t* counters;
+  uint32_t* errorHost;  // queue mode: pinned host word the last workgroup copies the error flags to (or null)
   uint32_t itemBegin, itemEnd, numSlots, spinLimit;
 };
 hipError_t launchBatch(hipStream_t stream, uint32_t format, uint32_t arithmetic, bool queue, const BatchArgs& args,
@@ -151,9 +152,6 @@ uint32_t batchMaxWorkgroupsPerCU(uint32_t format, uint32_t arithmetic, bool queu
 // measured to block the calling thread for about 10 ms now and then while
 // other streams had work in flight.
 hipError_t launchStageCopy(hipStream_t stream, void* dst, const void* src, size_t copyBytes, size_t totalBytes);
-// Copies one device word to pinned host memory in a one-wave kernel on
-// `stream` (for a batch's error flags: cheaper than a device-to-host copy).
-hipError_t launchReadWord(hipStream_t stream, uint32_t* hostDst, const uint32_t* devSrc);
 
 // Seeded synthetic level 0 (synth.hip): one image per job; a batched launch
 // writes chunks of kSynthChunk texels of many jobs.
