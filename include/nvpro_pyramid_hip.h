@@ -240,7 +240,10 @@ typedef struct NvproPyramidBatchConfig {
                                the rest of its image's plan, run in one workgroup; 0: never;
                                at most 128 */
   uint32_t useGraph;        /* nonzero: nvproPyramidBatchLaunch replays a hipGraph built at creation */
-  uint32_t reserved[3];     /* must be 0 */
+  uint32_t spinLimit;       /* queue mode: polls (each >= 50 ns) before a dependency wait gives up and
+                               sets NVPRO_PYRAMID_BATCH_ERROR_SPIN; 0: 2^20.  Small values are for
+                               tests of the error path. */
+  uint32_t reserved[2];     /* must be 0 */
 } NvproPyramidBatchConfig;
 
 typedef struct NvproPyramidBatchInfo {

@@ -199,3 +199,35 @@ def test_synthesize_rgba16f_matches_spec(w, h, stream):
         assert np.array_equal(got, want)
     finally:
         c.free()
+
+
+def test_batch_spin_limit_reports_error(stream):
+    """The error path of queue mode, forced from the host: with a spin limit of
+    one poll, a step's items give up waiting for the previous step of their
+    image (which is still running) and the launch reports
+    NVPRO_PYRAMID_BATCH_ERROR_SPIN; every wait is bounded, so the launch still
+    ends.  A batch of the same images with the default limit reports nothing
+    and matches the plain dispatches."""
+    sizes = [(2048, 2048), (2048, 1024), (1024, 2048), (1500, 1500)]
+    chains = _chains(sizes, P.FORMAT_SRGBA8, 300, stream)
+    try:
+        want = _plain_outputs(chains, stream)
+        images = [c.image for c in chains]
+        bad = P.PyramidBatch(P.pipelines(), images,
+                             config=P.batch_config(mode=P.BATCH_MODE_QUEUE, item_bytes=4096, use_graph=False,
+                                                   spin_limit=1))
+        try:
+            bad.launch(stream)
+            hip_rt.stream_sync(stream)
+            assert bad.read_errors() & P.BATCH_ERROR_SPIN
+        finally:
+            bad.destroy()
+        good = P.PyramidBatch(P.pipelines(), images, config=P.batch_config(mode=P.BATCH_MODE_QUEUE))
+        try:
+            got = _batch_outputs(good, chains, stream)
+        finally:
+            good.destroy()
+        for (w, h), a, b in zip(sizes, got, want):
+            assert np.array_equal(a, b), f"{w}x{h}: batch differs from nvproPyramidDispatch"
+    finally:
+        _free(chains)

@@ -49,11 +49,13 @@ def _drain(srv, tickets, timeout=60.0):
     return done
 
 
-@pytest.mark.parametrize("arith", [P.ARITH_FP32, P.ARITH_PACKED_FP16])
-def test_alone_equals_in_mix(arith, stream):
+@pytest.mark.parametrize("arith,mode", [(P.ARITH_FP32, P.BATCH_MODE_ROUNDS), (P.ARITH_PACKED_FP16, P.BATCH_MODE_ROUNDS),
+                                       (P.ARITH_FP32, P.BATCH_MODE_QUEUE)], ids=["fp32", "packed", "fp32-queue"])
+def test_alone_equals_in_mix(arith, mode, stream):
     sizes = synth.mixed_sizes(40, 21, hi=1024)
     fmts = [P.FORMAT_RGBA16F if k % 3 == 1 else P.FORMAT_SRGBA8 for k in range(len(sizes))]
-    srv = P.PyramidServer(P.server_config(max_in_flight=3, max_batch_images=7, arithmetic16f=arith))
+    srv = P.PyramidServer(P.server_config(max_in_flight=3, max_batch_images=7, arithmetic16f=arith,
+                                          batch=P.batch_config(mode=mode, use_graph=False)))
     try:
         tickets = {}
         # Arrivals in waves, polled in between, so batches form from whatever has arrived.
@@ -110,5 +112,25 @@ def test_admission_by_budget(stream):
         st = srv.stats()
         assert st.completed == 10 and st.released == 10 and st.residentBytes == 0
         assert st.peakResidentBytes == budget
+    finally:
+        srv.destroy()
+
+
+def test_failed_batch_is_reported(stream):
+    """Queue mode with a spin limit of one poll (forced from the host): the
+    dependency waits give up, the batch's error flags reach the host through
+    the kernel's exit, and every request of such a batch completes with a
+    nonzero status instead of hanging."""
+    cfg = P.server_config(max_in_flight=2, batch=P.batch_config(mode=P.BATCH_MODE_QUEUE, item_bytes=4096,
+                                                                use_graph=False, spin_limit=1))
+    srv = P.PyramidServer(cfg)
+    try:
+        tickets = [srv.submit(P.FORMAT_SRGBA8, 2048, 2048, seed=70 + k) for k in range(4)]
+        done = _drain(srv, tickets)
+        st = srv.stats()
+        assert st.failedBatches >= 1
+        assert any(c.status != 0 for c in done.values())
+        for t in tickets:
+            srv.release(t)
     finally:
         srv.destroy()

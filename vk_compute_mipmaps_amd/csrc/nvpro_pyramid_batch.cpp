@@ -51,7 +51,7 @@ int nvpro_pyramid::checkBatchConfig(const NvproPyramidBatchConfig& c, const char
     return hostFail(hipErrorInvalidValue, "%s: unknown mode %u", who, c.mode);
   if(c.tailTexels > kBatchTailTexels)
     return hostFail(hipErrorInvalidValue, "%s: tailTexels %u above %u", who, c.tailTexels, kBatchTailTexels);
-  if(c.reserved[0] || c.reserved[1] || c.reserved[2])
+  if(c.reserved[0] || c.reserved[1])
     return hostFail(hipErrorInvalidValue, "%s: reserved config fields must be 0", who);
   return 0;
 }
@@ -235,7 +235,7 @@ hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& p
   args_.counters  = (uint32_t*)(base + oCnt);
   extra_          = extra ? base + oExt : nullptr;
   args_.numSlots  = p.numSlots;
-  args_.spinLimit = kSpinLimit;
+  args_.spinLimit = cfg.spinLimit ? cfg.spinLimit : kSpinLimit;
   format_         = pipes.format;
   arithmetic_     = pipes.arithmetic;
   queue_          = cfg.mode == NVPRO_PYRAMID_BATCH_MODE_QUEUE;

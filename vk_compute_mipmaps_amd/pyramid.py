@@ -110,7 +110,7 @@ class NvproPyramidStep(ctypes.Structure):
 class NvproPyramidBatchConfig(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("workgroupsPerCU", ctypes.c_uint32),
                 ("itemBytes", ctypes.c_uint32), ("tailTexels", ctypes.c_uint32),
-                ("useGraph", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("useGraph", ctypes.c_uint32), ("spinLimit", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
     def as_dict(self):
         return {"mode": "queue" if self.mode == BATCH_MODE_QUEUE else "rounds",
@@ -442,11 +442,12 @@ class DeviceMipChain:
 
 
 def batch_config(mode: int = BATCH_MODE_QUEUE, workgroups_per_cu: int = 0, item_bytes: int = 65536,
-                 tail_texels: int = BATCH_MAX_TAIL_TEXELS, use_graph: bool = True) -> NvproPyramidBatchConfig:
+                 tail_texels: int = BATCH_MAX_TAIL_TEXELS, use_graph: bool = True,
+                 spin_limit: int = 0) -> NvproPyramidBatchConfig:
     c = NvproPyramidBatchConfig()
     _check(lib().nvproPyramidBatchDefaultConfig(ctypes.byref(c)))
     c.mode, c.workgroupsPerCU, c.itemBytes = mode, workgroups_per_cu, item_bytes
-    c.tailTexels, c.useGraph = tail_texels, 1 if use_graph else 0
+    c.tailTexels, c.useGraph, c.spinLimit = tail_texels, 1 if use_graph else 0, spin_limit
     return c
 
 
