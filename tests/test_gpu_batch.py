@@ -176,7 +176,8 @@ def test_batch_autotune(stream):
         _free(chains)
 
 
-@pytest.mark.parametrize("w,h", [(1, 1), (67, 3), (256, 256), (1000, 333)])
+# Widths below, at and above the generator's 256-thread stride, and 1-wide.
+@pytest.mark.parametrize("w,h", [(1, 1), (67, 3), (256, 256), (1000, 333), (1, 700), (5, 1000), (4097, 3)])
 def test_synthesize_srgba8_matches_spec(w, h, stream):
     c = P.DeviceMipChain(w, h, P.FORMAT_SRGBA8)
     try:
@@ -188,7 +189,7 @@ def test_synthesize_srgba8_matches_spec(w, h, stream):
         c.free()
 
 
-@pytest.mark.parametrize("w,h", [(1, 1), (67, 3), (300, 200)])
+@pytest.mark.parametrize("w,h", [(1, 1), (67, 3), (300, 200), (1, 300), (257, 20)])
 def test_synthesize_rgba16f_matches_spec(w, h, stream):
     c = P.DeviceMipChain(w, h, P.FORMAT_RGBA16F)
     try:

@@ -110,6 +110,34 @@ __device__ __forceinline__ uint32_t synthSrgba8TexelCached(const uint64_t (&sm)[
   return texel | (uint32_t)(hash4(sm[7], x, y) >> 56) << 24;
 }
 
+// Row-major texel t = y * w + x, walked by a workgroup: each thread starts at
+// first + threadIdx.x and steps by blockDim.x, with (x, y) kept incrementally
+// (one division per thread and chunk; a step adds q rows and r columns, where
+// blockDim.x = q * w + r).  A level 0 has fewer than 2^32 texels.
+struct TexelWalk
+{
+  uint32_t x, y, q, r, w;
+  __device__ TexelWalk(uint32_t first, uint32_t width)
+  {
+    const uint32_t t = first + threadIdx.x;
+    w                = width;
+    y                = t / width;
+    x                = t - y * width;
+    q                = blockDim.x / width;
+    r                = blockDim.x - q * width;
+  }
+  __device__ void step()
+  {
+    x += r;
+    y += q;
+    if(x >= w)
+    {
+      x -= w;
+      ++y;
+    }
+  }
+};
+
 // Texels [first, end) of `job` by the whole workgroup (uniform arguments;
 // contains barriers).
 __device__ __forceinline__ void synthTexels(LatticeCache& lat, const SynthJob& job, uint64_t first, uint64_t end,
@@ -130,23 +158,21 @@ __device__ __forceinline__ void synthTexels(LatticeCache& lat, const SynthJob& j
         lat.v[c][k]       = (uint8_t)(hash4(sm[c], ii, j0 + jj) >> 56);
       }
       __syncthreads();
-      for(uint64_t t = first + threadIdx.x; t < end; t += blockDim.x)
-      {
-        const uint32_t y = (uint32_t)(t / w), x = (uint32_t)(t - (uint64_t)y * w);
-        *reinterpret_cast<uint32_t*>(job.base + (uint64_t)y * job.pitch + (uint64_t)x * 4u) =
-            synthSrgba8TexelCached(sm, lat, ni, j0, x, y);
-      }
+      TexelWalk p((uint32_t)first, w);
+      for(uint64_t t = first + threadIdx.x; t < end; t += blockDim.x, p.step())
+        *reinterpret_cast<uint32_t*>(job.base + (uint64_t)p.y * job.pitch + (uint64_t)p.x * 4u) =
+            synthSrgba8TexelCached(sm, lat, ni, j0, p.x, p.y);
       return;
     }
   }
-  for(uint64_t t = first + threadIdx.x; t < end; t += blockDim.x)
+  TexelWalk p((uint32_t)first, w);
+  for(uint64_t t = first + threadIdx.x; t < end; t += blockDim.x, p.step())
   {
-    const uint32_t y = (uint32_t)(t / w), x = (uint32_t)(t - (uint64_t)y * w);
-    uint8_t* p       = job.base + (uint64_t)y * job.pitch;
+    uint8_t* row = job.base + (uint64_t)p.y * job.pitch;
     if(job.format == NVPRO_PYRAMID_FORMAT_SRGBA8)
-      *reinterpret_cast<uint32_t*>(p + (uint64_t)x * 4u) = synthSrgba8Texel(sm, x, y);
+      *reinterpret_cast<uint32_t*>(row + (uint64_t)p.x * 4u) = synthSrgba8Texel(sm, p.x, p.y);
     else
-      *reinterpret_cast<uint2*>(p + (uint64_t)x * 8u) = synthRgba16fTexel(sm, x, y);
+      *reinterpret_cast<uint2*>(row + (uint64_t)p.x * 8u) = synthRgba16fTexel(sm, p.x, p.y);
   }
 }
 
