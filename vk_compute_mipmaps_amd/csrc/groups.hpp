@@ -374,6 +374,48 @@ __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel,
   }
 }
 
+// The second level of a two-level tile when its horizontal kernel is 3 wide
+// (KY2: its vertical kernel size when known at compile time, else 0 and
+// `ky2`).  Threads form rows of 32 lanes, TX outputs and a helper; each lane
+// takes the vertical taps of mid columns 2 ox and 2 ox + 1 and receives column
+// 2 ox + 2 from lane + 1 (its column 2 (ox + 1)); the helper (ox = TX) only
+// lends its column.  Per output these are generalSample's operations on the
+// same values, from six LDS reads instead of nine.
+template <class F, bool CO, int KY2, class TT>
+__device__ __forceinline__ void gen2SecondLevel3(const TT& tab, const typename F::Val* mid, const ImageDesc& img,
+                                                 uint32_t inLevel, uint32_t tilesX, uint32_t tile, int ky2)
+{
+  using G = Gen2Geom<F::kBytes>;
+  using V = typename F::Val;
+  static_assert(G::TX + 1u == 32u, "a row of outputs and its helper fill half a wave");
+  const int ky = KY2 ? KY2 : ky2;
+  if(threadIdx.x < 32u * G::TY)  // whole half-waves: the lane shift sees every lane of a row
+  {
+    const LevelDesc& l2 = img.lv[inLevel + 2u];
+    const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
+    const uint32_t ox = threadIdx.x & 31u, oy = threadIdx.x >> 5;
+    const uint32_t x = G::TX * tx + ox, y = G::TY * ty + oy;
+    const Weights3 wy = weights3(axis3(l2.height), y);
+    const uint32_t c0 = 2u * ox, c1 = c0 + 1u < G::MW ? c0 + 1u : c0;  // the helper's second column is past the row
+    V h[2];
+#pragma unroll
+    for(int j = 0; j < 2; ++j)
+    {
+      const V* col = mid + (2u * oy) * G::MW + (j ? c1 : c0);
+      const V a0   = col[0];
+      const V a1   = ky >= 2 ? col[G::MW] : a0;
+      const V a2   = ky == 3 ? col[2u * G::MW] : a0;
+      h[j]         = verticalTap<F>(wy, ky, a0, a1, a2);
+    }
+    const V h2 = waveShiftDown1(h[0]);
+    if(ox < G::TX && x < l2.width && y < l2.height)
+    {
+      const V v = horizontalTap<F>(axis3(l2.width), x, 3, h[0], h[1], h2);
+      storeTexelTo<F, CO>(texelPtr(img, inLevel + 2u, x, y, F::kBytes), F::store(tab, v));
+    }
+  }
+}
+
 // Contains barriers.  CO: the second level is stored with storeTexelCoherent.
 template <class F, bool CO = false, int KX = 0, int KY = 0, class TT>
 __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid, const ImageDesc& img,
@@ -423,7 +465,16 @@ __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid,
     }
   }
   __syncthreads();
-  if(threadIdx.x < G::TX * G::TY)
+  if(kx2 == 3)  // the same for every tile of a step
+  {
+    if(ky2 == 3)
+      gen2SecondLevel3<F, CO, 3>(tab, mid, img, inLevel, tilesX, tile, ky2);
+    else if(ky2 == 2)
+      gen2SecondLevel3<F, CO, 2>(tab, mid, img, inLevel, tilesX, tile, ky2);
+    else
+      gen2SecondLevel3<F, CO, 0>(tab, mid, img, inLevel, tilesX, tile, ky2);
+  }
+  else if(threadIdx.x < G::TX * G::TY)
   {
     const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
     const uint32_t ox = threadIdx.x % G::TX, oy = threadIdx.x / G::TX;
