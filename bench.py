@@ -290,8 +290,10 @@ def serving_leg():
     return out
 
 
-def large_configs(stream, reps=20):
-    """Single chains of the other configurations: median us and algorithmic GB/s."""
+def large_configs(stream, reps=50, warmup=10):
+    """Single chains of the other configurations: median us and algorithmic GB/s
+    over `reps` back-to-back chains after `warmup` untimed ones (a short burst
+    from idle runs the large chains up to 10 % slower)."""
     from vk_compute_mipmaps_amd import hip_rt, pyramid as P
 
     res = {}
@@ -306,7 +308,8 @@ def large_configs(stream, reps=20):
                                           P.ARITH_PACKED_FP16, 7)):
         c = P.DeviceMipChain(w, h, fmt)
         c.synthesize(seed, stream)
-        c.dispatch(stream, arithmetic=arith)
+        for _ in range(warmup):
+            c.dispatch(stream, arithmetic=arith)
         hip_rt.stream_sync(stream)
         evs = [hip_rt.event_create() for _ in range(reps + 1)]
         hip_rt.event_record(evs[0], stream)

@@ -134,6 +134,26 @@ static uint64_t firstOutput(const ImageDesc& img, const NvproPyramidStep& s)
   return (uint64_t)first.width * first.height;
 }
 
+// Persistent workgroups per compute unit of a fast step of `groups` groups
+// (measured on MI355X, sRGBA8 F(0,6), per-step time): up to 16 groups per
+// compute unit, 8 (the most that fit: 4096^2 22.3 us; with 4 or 3, 22.6-22.8);
+// fewer workgroups on larger steps, although each then has fewer waves to
+// hide latency behind: 5120^2 (25 groups per unit) 33.2 -> 32.6 us with 4,
+// 6144^2 (36) 47.1 -> 45.2 with 3 (44.8 with 4), 8192x4096 (32) 43.0 -> 39.8 with 3, 8192^2
+// (64) 86.8 -> 76.7 with 3, 16384^2 (256) 350.1 -> 296.0 with 3 (with 2:
+// 344.3).
+static uint32_t fastWorkgroupsPerCU(uint64_t groups)
+{
+  const uint64_t cus = deviceComputeUnits();
+  return groups >= 32u * cus ? 3u : groups > 16u * cus ? 4u : 8u;
+}
+
+static uint32_t fastWorkgroupsPerCU(const ImageDesc& img, const NvproPyramidStep& s)
+{
+  const LevelDesc& in = img.lv[s.inputLevel];
+  return fastWorkgroupsPerCU(fastGroupCount(in.width, in.height, s.levelCount));
+}
+
 // Whether fusing a successor into fast step s pays, by the step's workgroups
 // (measured on MI355X, sRGBA8): a small grid counts at one arrival counter and
 // gains the successor's launch and start-up (4 workgroups with a one-texel
@@ -146,7 +166,8 @@ static bool fusionPays(const ImageDesc& img, const NvproPyramidStep& s)
 {
   const LevelDesc& in   = img.lv[s.inputLevel];
   const uint64_t groups = fastGroupCount(in.width, in.height, s.levelCount);
-  const uint64_t blocks = groups < 8ull * deviceComputeUnits() ? groups : 8ull * deviceComputeUnits();
+  const uint64_t cap    = (uint64_t)fastWorkgroupsPerCU(groups) * deviceComputeUnits();
+  const uint64_t blocks = groups < cap ? groups : cap;
   return blocks <= kArriveLeaves || blocks >= 4ull * deviceComputeUnits();
 }
 
@@ -175,7 +196,7 @@ hipError_t executePlan(hipStream_t stream, const ImageDesc& img, const NvproPyra
       {
         const uint32_t tilesX = nin.width >> n.levelCount, tilesY = nin.height >> n.levelCount;
         const NextStep next{arrive, kNextFast, n.inputLevel, n.levelCount, tilesX, tilesX * tilesY};
-        if(hipError_t e = launchFast(stream, img, s.inputLevel, s.levelCount, 8u, &next))
+        if(hipError_t e = launchFast(stream, img, s.inputLevel, s.levelCount, fastWorkgroupsPerCU(img, s), &next))
           return e;
         ++i;
         continue;
@@ -189,10 +210,10 @@ hipError_t executePlan(hipStream_t stream, const ImageDesc& img, const NvproPyra
         const TailPlan tail = tailFrom(steps, i + 1u, count);
         const NextStep next{arrive, kNextTail, 0u, 0u, 0u, 0u};
         ++*launches;
-        return launchFast(stream, img, s.inputLevel, s.levelCount, 8u, &next, &tail);
+        return launchFast(stream, img, s.inputLevel, s.levelCount, fastWorkgroupsPerCU(img, s), &next, &tail);
       }
     }
-    hipError_t e = fast ? launchFast(stream, img, s.inputLevel, s.levelCount, 8u)
+    hipError_t e = fast ? launchFast(stream, img, s.inputLevel, s.levelCount, fastWorkgroupsPerCU(img, s))
                         : launchGeneral(stream, img, s.inputLevel, s.levelCount);
     if(e != hipSuccess)
       return e;
