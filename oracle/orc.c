@@ -131,8 +131,10 @@ double orc_srgb_decode_f64(uint32_t code)
 /* FP32 rounding of the exact decode. */
 float orc_srgb_decode_f32(uint32_t code) { return (float)orc_srgb_decode_f64(code); }
 
-/* shaders/srgb.h:216: arg.a * (1.0f / 255.0f). */
-float orc_alpha_decode_f32(uint32_t code) { return (float)code * (1.0f / 255.0f); }
+/* Alpha in code units: the UNORM decode c / 255 of shaders/srgb.h:55-61 with
+ * the factor 255 moved into the encode (a linear scale commutes with every
+ * reduction of the schedule). */
+float orc_alpha_decode_f32(uint32_t code) { return (float)code; }
 
 /* srgba8_mipmap_preamble.glsl:110-121 in exact arithmetic (double is exact for
  * every float input to well below one code; pinned by the threshold tests). */
@@ -149,10 +151,11 @@ uint32_t orc_srgb_encode_exact(double x)
   return (uint32_t)floor(u);
 }
 
-/* srgba8_mipmap_preamble.glsl:125 with the multiply-add contracted. */
+/* srgba8_mipmap_preamble.glsl:125, uint(a * 255 + 0.5) clamped, on a value in
+ * code units: floor(a + 0.5). */
 uint32_t orc_alpha_encode_f32(float a)
 {
-  float t = fmaf(a, 255.0f, 0.5f);
+  float t = a + 0.5f;
   if(!(t > 0.0f))
     return 0u;
   if(t >= 255.0f)
@@ -162,7 +165,7 @@ uint32_t orc_alpha_encode_f32(float a)
 
 uint32_t orc_alpha_encode_f64(double a)
 {
-  double t = a * 255.0 + 0.5;
+  double t = a + 0.5;
   if(!(t > 0.0))
     return 0u;
   if(t >= 255.0)
@@ -462,7 +465,7 @@ int orc_pyramid_srgba8_f64(uint8_t* chain, uint32_t w, uint32_t h, uint32_t mipL
     {
       for(int c = 0; c < 3; ++c)
         prev[i].c[c] = g_decode_f64[in[4 * i + c]];
-      prev[i].c[3] = in[4 * i + 3] / 255.0;
+      prev[i].c[3] = (double)in[4 * i + 3];
     }
     for(uint32_t r = 1; r <= count; ++r)
     {

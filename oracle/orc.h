@@ -18,9 +18,12 @@
  * Arithmetic the reference leaves to the Vulkan implementation is fixed here as
  * follows (each choice is inside the reference's implementation-defined range):
  *   - sRGB decode of a code c: FP32 rounding of the exact value of srgb.h:18-28.
- *   - alpha decode: c * (1.0f / 255.0f) (srgb.h:216).
+ *   - alpha in code units: the reference decodes c / 255 (srgb.h:55-61) and
+ *     encodes uint(a * 255 + 0.5) (preamble:123-128), a linear scale that
+ *     commutes with every reduction; here the value is c itself and the encode
+ *     floor(a + 0.5) clamped, so box filters of alpha are exact and .5 ties
+ *     round up as in exact arithmetic.
  *   - sRGB encode: the exact real value of preamble:110-121 (no pow error).
- *   - alpha encode: floor(fmaf(a, 255, 0.5)) clamped (preamble:125, contracted).
  *   - 3-tap REDUCE (preamble:24-25): fmaf(w2, v2, fmaf(w1, v1, w0 * v0)).
  *   - reciprocal 1 / (2n + 1) (glsl:583): IEEE division.
  *   - LOAD_REDUCE4 without the bilinear sampler (preamble:40 with

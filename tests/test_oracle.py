@@ -109,15 +109,21 @@ def test_roundtrip_every_code(orc):
     L = orc.lib()
     for c in range(256):
         assert L.orc_srgb_encode_exact(L.orc_srgb_decode_f32(c)) == c
+        assert L.orc_alpha_decode_f32(c) == float(c)  # code units: exact
         assert L.orc_alpha_encode_f32(L.orc_alpha_decode_f32(c)) == c
-        assert L.orc_alpha_encode_f64(c / 255.0) == c
+        assert L.orc_alpha_encode_f64(float(c)) == c
 
 
 def test_alpha_encode_rounds_half_up(orc):
+    """uint(a * 255 + 0.5) clamped (srgba8_mipmap_preamble.glsl:125) on a value
+    already in code units: floor(a + 0.5), ties up, clamped to [0, 255]."""
     L = orc.lib()
-    assert L.orc_alpha_encode_f64((10 + 0.5) / 255.0) == 11
-    assert L.orc_alpha_encode_f64((10 + 0.49) / 255.0) == 10
-    assert L.orc_alpha_encode_f32(0.0) == 0 and L.orc_alpha_encode_f32(1.0) == 255
+    assert L.orc_alpha_encode_f64(10.5) == 11
+    assert L.orc_alpha_encode_f64(10.49) == 10
+    assert L.orc_alpha_encode_f32(10.5) == 11 and L.orc_alpha_encode_f32(10.25) == 10
+    assert L.orc_alpha_encode_f32(254.5) == 255 and L.orc_alpha_encode_f32(300.0) == 255
+    assert L.orc_alpha_encode_f32(0.0) == 0 and L.orc_alpha_encode_f32(-3.0) == 0
+    assert L.orc_alpha_encode_f32(255.0) == 255
 
 
 def test_product_tables_match_oracle(orc):
@@ -336,6 +342,69 @@ def test_constant_images_stay_constant(w, h, orc):
         assert (ref[:, :3] == code).all()
 
 
+def alpha_by_integer_sums(alpha0, steps):
+    """Alpha codes of every level of a square power-of-2 chain in integer
+    arithmetic: inside a step, level L + r is the mean of the 2^r x 2^r block of
+    the step's input codes, rounded half up (uint(a * 255 + 0.5) of
+    srgba8_mipmap_preamble.glsl:125 on the exact value); steps start from the
+    codes the previous step stored."""
+    levels = [alpha0.astype(np.int64)]
+    for s in steps:
+        src = levels[s.inputLevel]
+        for r in range(1, s.levelCount + 1):
+            n = src.shape[0] >> r
+            b = 1 << r
+            total = src.reshape(n, b, n, b).sum(axis=(1, 3))
+            levels.append((total + (b * b >> 1)) // (b * b))
+    return levels
+
+
+@pytest.mark.parametrize("fast", [True, False])
+def test_alpha_is_exact_on_box_filters(fast, orc):
+    """Pins the alpha channel of both emulators: with alpha in code units the 2x2
+    boxes are exact in FP32 and FP64, so every level equals the integer
+    block mean, .5 ties rounded up."""
+    w = h = 1024
+    l0 = H.level0_srgba8(w, h, 1)
+    chain = H.host_chain(w, h, l0)
+    steps = orc.plan(w, h, 0, fast)
+    want = alpha_by_integer_sums(l0[..., 3], steps)
+    ties = 0
+    for precision in ("f32", "f64"):
+        out = orc.pyramid_srgba8(chain, w, h, steps, 0, precision)
+        for l in range(1, P.full_level_count(w, h)):
+            got = H.level_view(out, w, h, l)[..., 3]
+            np.testing.assert_array_equal(got, want[l], err_msg=f"{precision} level {l}")
+    # The case this settles: a quarter of level 1's texels are exact .5 ties.
+    s = l0[..., 3].astype(np.int64).reshape(512, 2, 512, 2).sum(axis=(1, 3))
+    ties = int((s % 4 == 2).sum())
+    assert ties > 0.2 * s.size
+
+
+def test_srgba8_f64_matches_numpy(orc):
+    """The FP64 oracle's 2x2 path against an independent numpy evaluation: decode
+    with the formula of shaders/srgb.h:18-28 in float64, block means over one
+    fast step (64^2: F(0,6), no re-quantisation inside), encode with
+    srgba8_mipmap_preamble.glsl:110-121 in float64; alpha as integer sums."""
+    w = h = 64
+    l0 = H.level0_srgba8(w, h, 21)
+    chain = H.host_chain(w, h, l0)
+    steps = orc.plan(w, h)
+    assert [(s.pipeline, s.inputLevel, s.levelCount) for s in steps] == [(1, 0, 6)]
+    out = orc.pyramid_srgba8(chain, w, h, steps, 0, "f64")
+    u = l0[..., :3].astype(np.float64) / 255.0
+    lin = np.where(u <= 0.04045, u * (25.0 / 323.0), ((200.0 * u + 11.0) / 211.0) ** 2.4)
+    alpha = alpha_by_integer_sums(l0[..., 3], steps)
+    for r in range(1, 7):
+        n, b = w >> r, 1 << r
+        m = lin.reshape(n, b, n, b, 3).mean(axis=(1, 3))
+        srgb = np.where(m <= 0.0031308, 12.92 * m, 1.055 * np.power(m, 1.0 / 2.4) - 0.055)
+        want = np.clip(np.floor(srgb * 255.0 + 0.5), 0, 255).astype(np.uint8)
+        got = H.level_view(out, w, h, r)
+        np.testing.assert_array_equal(got[..., :3], want, err_msg=f"rgb level {r}")
+        np.testing.assert_array_equal(got[..., 3], alpha[r], err_msg=f"alpha level {r}")
+
+
 # ---------------------------------------------------------------------------
 # Emulators against each other and against the reference CPU generator
 # ---------------------------------------------------------------------------
@@ -347,9 +416,10 @@ def test_fp32_emulator_within_one_lsb_of_fp64(w, h, seed, orc):
     steps = orc.plan(w, h)
     a = orc.pyramid_srgba8(chain, w, h, steps, 0, "f32")
     b = orc.pyramid_srgba8(chain, w, h, steps, 0, "f64")
-    worst, rgb_exact, _ = H.rgb_alpha_stats(a, b, w, h)
+    worst, rgb_exact, texel_exact = H.rgb_alpha_stats(a, b, w, h)
     assert worst <= 1
     assert rgb_exact >= 0.999
+    assert texel_exact >= 0.999, f"{w}x{h}: {texel_exact:.5f} of texels equal the FP64 oracle"
 
 
 @pytest.mark.parametrize("w,h,seed", [(1024, 1024, 1), (1920, 1080, 4), (4095, 2047, 3), (2047, 2047, 13)])

@@ -24,10 +24,11 @@ __global__ void __launch_bounds__(256) premultiplySrgba8Kernel(uint8_t* base, ui
     const uint32_t y = (uint32_t)(t / width), x = (uint32_t)(t - (uint64_t)y * width);
     uint32_t* p      = reinterpret_cast<uint32_t*>(base + (uint64_t)y * pitch + (uint64_t)x * 4u);
     const uint32_t c = *p;
-    const float4 v   = Srgba8::load<false>(tab, c);  // linear RGB, alpha = code / 255
-    const uint32_t r = Srgba8::encodeChannel(tab, v.x * v.w);
-    const uint32_t g = Srgba8::encodeChannel(tab, v.y * v.w);
-    const uint32_t b = Srgba8::encodeChannel(tab, v.z * v.w);
+    const float4 v   = Srgba8::load<false>(tab, c);  // linear RGB
+    const float a    = (float)(c >> 24) * (1.0f / 255.0f);
+    const uint32_t r = Srgba8::encodeChannel(tab, v.x * a);
+    const uint32_t g = Srgba8::encodeChannel(tab, v.y * a);
+    const uint32_t b = Srgba8::encodeChannel(tab, v.z * a);
     *p               = r | g << 8 | b << 16 | (c & 0xff000000u);
   }
 }

@@ -124,12 +124,18 @@ struct Srgba8 : Fp32Arith
   using Raw                        = uint32_t;
   static constexpr uint32_t kBytes = 4;
 
-  // LDS tables: the decoded value of every code for RGB and for alpha (both
-  // times the fill scale), and the encode buckets.
+  // Alpha is carried in code units: the reference decodes the UNORM code as
+  // c / 255 and encodes uint(a * 255 + 0.5) (preamble:125), a linear map whose
+  // scale commutes with every reduction, so the value here is c itself (times
+  // the fill scale) and the encode is floor(a + 0.5).  Box filters of codes are
+  // then exact in FP32 (a 2x2 sum of 4^k codes stays below 2^24), and every
+  // .5 tie rounds up as exact arithmetic does.
+  //
+  // LDS tables: the decoded value of every RGB code (times the fill scale) and
+  // the encode buckets.
   struct Tables
   {
     float dec[256];
-    float adec[256];
     uint2 enc[NVPRO_SRGB_ENC_BUCKETS];  // {first float with a higher code, code at start}
   };
 
@@ -158,10 +164,7 @@ struct Srgba8 : Fp32Arith
     {
       const uint32_t i = threadIdx.x + k * BS;
       if(i < 256u)
-      {
-        t.dec[i]  = d[k] * scale;
-        t.adec[i] = (float)i * (1.0f / 255.0f) * scale;  // shaders/srgb.h:216, then the exact scale
-      }
+        t.dec[i] = d[k] * scale;
     }
 #pragma unroll
     for(int k = 0; k < kEnc; ++k)
@@ -188,17 +191,13 @@ struct Srgba8 : Fp32Arith
   }
 
   // texelFetch of the sRGB sampler (preamble:21-22): sRGB decode of RGB by table
-  // lookup and the alpha code * (1/255) (shaders/srgb.h:216).  With plain
-  // Tables the fill scale applies whatever PRE says: PRE loads (the fast
-  // pipeline, tables filled at 0.25) look alpha up too; plain loads (tables
-  // filled at 1) compute it, the same single rounding as the table entry, which
-  // takes a quarter of the LDS reads off the VALU-bound general kernels
-  // (measured: 4095^2 chain 51.7 -> 50.1 us; the fast kernel was 1.5 % slower
-  // with it).
+  // lookup; alpha in code units (see Tables).  With plain Tables, PRE says the
+  // tables were filled at 0.25 (the fast pipeline) and alpha gets the same
+  // exact scale; plain loads (tables filled at 1) take the code as it is.
   template <bool PRE>
   __device__ static float4 load(const Tables& t, uint32_t texel)
   {
-    const float a = PRE ? t.adec[texel >> 24] : (float)(texel >> 24) * (1.0f / 255.0f);
+    const float a = (float)(texel >> 24) * (PRE ? 0.25f : 1.0f);
     return make_float4(t.dec[texel & 0xffu], t.dec[(texel >> 8) & 0xffu], t.dec[(texel >> 16) & 0xffu], a);
   }
   template <bool PRE>
@@ -219,10 +218,11 @@ struct Srgba8 : Fp32Arith
     return e.y + (x >= __uint_as_float(e.x) ? 1u : 0u);
   }
 
-  // srgbFromLinearVec (preamble:123-128); alpha = floor(fma(a, 255, 0.5)) clamped.
+  // srgbFromLinearVec (preamble:123-128); alpha (code units) = floor(a + 0.5)
+  // clamped to [0, 255].
   __device__ static uint32_t store(const Tables& t, float4 v)
   {
-    uint32_t a = (uint32_t)fminf(fmaxf(__builtin_fmaf(v.w, 255.0f, 0.5f), 0.0f), 255.0f);
+    uint32_t a = (uint32_t)fminf(fmaxf(v.w + 0.5f, 0.0f), 255.0f);
     return encodeChannel(t, v.x) | encodeChannel(t, v.y) << 8 | encodeChannel(t, v.z) << 16 | a << 24;
   }
 };
