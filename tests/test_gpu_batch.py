@@ -142,18 +142,76 @@ def test_batch_rgba16f_equals_plain(arith, mode, stream):
         _free(chains)
 
 
-def test_batch_mixed_workload_subset(stream):
-    """The first 48 textures of the 512-texture workload (seed 8) at full size."""
-    sizes = synth.mixed_sizes(512, 8)[:48]
+def _digest(a: np.ndarray) -> bytes:
+    import hashlib
+
+    return hashlib.blake2b(memoryview(a), digest_size=20).digest()
+
+
+def _mixed_sample(sizes, per_class=12):
+    """Indices of a sample of the mixed workload: both sides odd, both sides
+    multiples of 4 (plans that start with fast steps), and the other even
+    sizes; the smallest and largest of each class plus evenly spaced ones."""
+    classes = {
+        "odd": [k for k, (w, h) in enumerate(sizes) if w % 2 and h % 2],
+        "fast": [k for k, (w, h) in enumerate(sizes) if w % 4 == 0 and h % 4 == 0],
+    }
+    classes["even"] = [k for k in range(len(sizes)) if k not in classes["odd"] and k not in classes["fast"]]
+    out = []
+    for name, ks in classes.items():
+        ks = sorted(ks, key=lambda k: sizes[k][0] * sizes[k][1])
+        if len(ks) <= per_class:
+            out += ks
+        else:
+            out += [ks[round(i * (len(ks) - 1) / (per_class - 1))] for i in range(per_class)]
+    return sorted(set(out)), {name: len(ks) for name, ks in classes.items()}
+
+
+def test_batch_mixed_workload_512(stream):
+    """The whole 512-texture workload (sizes of synth.mixed_sizes(512, 8): each side
+    log-uniform in [64, 4096], 40 % odd) in one batch, queue and rounds modes:
+    every image byte-equal to its own nvproPyramidDispatch, and a sample of 36
+    (odd, fast-step and other even sizes) equal to the FP32 oracle."""
+    sizes = synth.mixed_sizes(512, 8)
+    assert len(sizes) == 512
     chains = _chains(sizes, P.FORMAT_SRGBA8, 8000, stream)
+    sample, counts = _mixed_sample(sizes)
+    print(f"512 textures, classes {counts}, oracle sample {len(sample)}", flush=True)
+    assert len(sample) >= 32 and min(counts.values()) > 0
     try:
-        want = _plain_outputs(chains, stream)
+        _poison(chains, stream)
+        for c in chains:
+            c.dispatch(stream=stream)
+        hip_rt.stream_sync(stream)
+        want, kept = [], {}
+        for k, c in enumerate(chains):
+            a = c.download()
+            want.append(_digest(a))
+            if k in sample:
+                kept[k] = a
         for cfg in (P.batch_config(), P.batch_config(mode=P.BATCH_MODE_ROUNDS)):
             batch = P.PyramidBatch(P.pipelines(), [c.image for c in chains], config=cfg)
-            got = _batch_outputs(batch, chains, stream)
-            for (w, h), a, b in zip(sizes, got, want):
-                assert np.array_equal(a, b), f"{w}x{h}: batch differs ({cfg.as_dict()})"
-            batch.destroy()
+            try:
+                info = batch.info()
+                assert info.imageCount == 512
+                _poison(chains, stream)
+                batch.launch(stream)
+                hip_rt.stream_sync(stream)
+                assert batch.read_errors() == 0
+            finally:
+                batch.destroy()
+            for k, c in enumerate(chains):
+                a = c.download()
+                assert _digest(a) == want[k], f"{sizes[k]}: batch differs from nvproPyramidDispatch ({cfg.as_dict()})"
+                if k in kept:
+                    assert np.array_equal(a, kept[k])
+        for k in sample:
+            w, h = sizes[k]
+            gpu = kept[k]
+            host = H.host_chain(w, h, gpu[: w * h * 4])  # level 0 as the device generated it
+            emu = H.emulate_srgba8(host, w, h, P.plan(w, h))
+            worst, where, ndiff, _ = orc.compare(gpu, emu, w, h)
+            assert worst == 0, f"{w}x{h}: {worst} LSB from the FP32 oracle at {where} ({ndiff} texels)"
     finally:
         _free(chains)
 

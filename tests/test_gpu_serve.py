@@ -134,3 +134,65 @@ def test_failed_batch_is_reported(stream):
             srv.release(t)
     finally:
         srv.destroy()
+
+
+def test_poisson_mix_at_stated_workload(stream):
+    """The serving workload: Poisson arrivals, sizes of the 512-texture
+    distribution (each side log-uniform in [64, 4096], 40 % odd), a 70/30 mix of
+    sRGBA8 and RGBA16F, an 80 GB admission budget.  Requests are submitted at
+    their arrival times while the loop polls, so batches form from whatever has
+    arrived.  Every served chain equals its alone result byte for byte, and a
+    sample of each format equals the FP32 oracle (orc_pyramid_srgba8 /
+    orc_pyramid_rgba16f) run on the served level 0."""
+    from oracle import orc
+    from tests import helpers as H
+
+    n, rate = 240, 4000.0
+    rng = np.random.default_rng(55)
+    sizes = synth.mixed_sizes(n, 55)
+    fmts = [P.FORMAT_SRGBA8 if u < 0.7 else P.FORMAT_RGBA16F for u in rng.random(n)]
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    srv = P.PyramidServer(P.server_config(budget_bytes=80_000_000_000))
+    try:
+        tickets, done = {}, {}
+        t0 = time.perf_counter()
+        k = 0
+        t_end = time.time() + 90
+        while len(done) < n:
+            now = time.perf_counter() - t0
+            while k < n and arrivals[k] <= now:
+                w, h = sizes[k]
+                tickets[srv.submit(fmts[k], w, h, seed=5000 + k)] = k
+                k += 1
+            for c in srv.poll():
+                done[c.ticket] = c
+            assert time.time() < t_end, f"{len(done)} of {n} requests completed"
+        st = srv.stats()
+        print(f"served {n} requests in {st.batches} batches, peak resident {st.peakResidentBytes / 1e9:.2f} GB",
+              flush=True)
+        assert st.failedBatches == 0 and st.batches < n  # some batches held several requests
+        by_format = {P.FORMAT_SRGBA8: [], P.FORMAT_RGBA16F: []}
+        for t, c in done.items():
+            k = tickets[t]
+            w, h = sizes[k]
+            assert c.status == 0
+            got = _download(c.image, c.bytes)
+            want = _alone(fmts[k], w, h, 5000 + k, stream)
+            assert np.array_equal(got, want), f"request {k} ({w}x{h}, format {fmts[k]}) differs from its alone result"
+            by_format[fmts[k]].append((k, got))
+            srv.release(t)
+        assert srv.stats().residentBytes == 0
+        for fmt, served in by_format.items():
+            assert len(served) >= 30
+            for k, got in served[:: len(served) // 6][:6]:
+                w, h = sizes[k]
+                steps = H.to_orc_steps(P.plan(w, h, fmt=fmt))
+                if fmt == P.FORMAT_SRGBA8:
+                    emu = orc.pyramid_srgba8(H.host_chain(w, h, got[: w * h * 4]), w, h, steps)
+                    worst, where, _, _ = orc.compare(got, emu, w, h)
+                    assert worst == 0, f"request {k} ({w}x{h}): {worst} LSB from the oracle at {where}"
+                else:
+                    emu = orc.pyramid_rgba16f(got.copy(), w, h, steps)
+                    np.testing.assert_array_equal(got.view(np.uint16), emu, err_msg=f"request {k} ({w}x{h})")
+    finally:
+        srv.destroy()
