@@ -178,6 +178,13 @@ def main():
     k_bytes = P.packed_layout(W, H, step_levels)[1]
     k_gbs = k_bytes / (k_us * 1e-6) / 1e9
 
+    # Speed of light: the same rotation through a pass that reads level 0 once
+    # and writes every level once without filtering (csrc/speed_of_light.hip,
+    # the reference's `baseline` pipeline's traffic).
+    sol = speed_of_light(stream, chains, n_buf, k_reps, evs[0], evs[1])
+    for c in chains:  # the pass wrote non-mip values: restore the poison
+        c.poison(stream=stream)
+
     # Achievable copy bandwidth for reference (device-to-device, 1 GiB).
     cp_bytes = 1 << 30
     src, dst = hip_rt.malloc(cp_bytes), hip_rt.malloc(cp_bytes)
@@ -197,21 +204,28 @@ def main():
     for c in chains:
         c.free()
     configs = None if args.no_configs else large_configs(stream)
+    # GPU outputs of the parity configurations (compared with the oracle in the
+    # CPU-baseline leg, outside every timed region).
+    parity_inputs = None if args.no_cpu_baseline else parity_outputs(stream)
     hip_rt.stream_destroy(stream)
     batched = None if args.no_batched else batched_leg()
     serving = None if args.no_serving else serving_leg()
 
-    traffic = None  # HBM bytes per F(0,6) launch, from the rocprofv3 --pmc summary
+    # HBM bytes per launch from the committed rocprofv3 --pmc summary: the
+    # chain's fused launch, and F(0,6) alone.
+    traffic = {}
     tpath = os.path.join(REPO, "profiles", "fast_pipeline_srgba8_4096.md")
     if os.path.exists(tpath):
         with open(tpath) as f:
             for line in f:
-                if line.startswith("- traffic_bytes_per_launch:"):
-                    traffic = int(line.split(":")[1].split()[0])
+                for key in ("chain_traffic_bytes_per_launch", "first_step_traffic_bytes_per_launch"):
+                    if line.startswith(f"- {key}:"):
+                        traffic[key] = int(line.split(":")[1].split()[0])
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(level0, args.cpu_seconds)
+        parity = parity_stats(parity_inputs)
 
     chain_total_bytes = chain_bytes
     out = {
@@ -242,22 +256,143 @@ def main():
         "chain_roofline_frac": round(chain_total_bytes / (us_per_chain * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {
             "bound": "hbm",
-            "kernel": "fastPipelineKernel<Srgba8, 6, false> (F(0,6) over level 0, launched alone)",
-            "achieved": round(k_gbs, 1),
+            "kernel": "fastPipelineKernel<Srgba8, 6, true> (the chain's one launch: F(0,6), then F(6,6) "
+                      "in its last workgroup)",
+            "achieved": round(chain_total_bytes / (us_per_chain * 1e-6) / 1e9, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(k_gbs / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel_us": round(k_us, 3),
-            "algorithmic_bytes_per_launch": k_bytes,
+            "frac": round(chain_total_bytes / (us_per_chain * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": traffic.get("chain_traffic_bytes_per_launch"),
+            "kernel_us": round(us_per_chain, 3),
+            "algorithmic_bytes_per_launch": chain_total_bytes,
+            "timing": "HIP events on the dispatch stream over the timed region; one launch per chain",
         },
+        "first_step_alone": {
+            "kernel": "fastPipelineKernel<Srgba8, 6, false> (F(0,6) over level 0, mipLevels 7)",
+            "us": round(k_us, 3),
+            "GBps": round(k_gbs, 1),
+            "frac": round(k_gbs / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes_per_launch": k_bytes,
+            "traffic": traffic.get("first_step_traffic_bytes_per_launch"),
+        },
+        "speed_of_light": sol,
+        "chain_vs_speed_of_light": round(us_per_chain / sol["us"], 3),
         "copy_GBps": round(copy_gbs, 1),
         "cpu_baseline": cpu,
+        "parity": parity,
         "batched": batched,
         "serving": serving,
         "configs": configs,
     }
     print(json.dumps(out))
+
+
+def speed_of_light(stream, chains, n_buf, reps, ev0, ev1):
+    """us per pass of csrc/speed_of_light.hip over the rotating 4096^2 chains:
+    level 0 read once, every level written once, no filtering (incorrect values
+    by design, like extras/general_pipelines/baseline/general_pipeline_alternative.glsl)."""
+    import ctypes
+
+    from vk_compute_mipmaps_amd import build, hip_rt, pyramid as P
+
+    sol = ctypes.CDLL(build.SOL_PATH)
+    sol.nvproPyramidSpeedOfLight.restype = ctypes.c_int
+    sol.nvproPyramidSpeedOfLight.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                             ctypes.c_uint32]
+
+    def run(i):
+        rc = sol.nvproPyramidSpeedOfLight(ctypes.c_void_p(stream), ctypes.c_void_p(chains[i % n_buf].ptr), W, H, 0)
+        if rc != 0:
+            raise RuntimeError(f"nvproPyramidSpeedOfLight failed ({rc})")
+
+    for i in range(20):
+        run(i)
+    hip_rt.stream_sync(stream)
+    hip_rt.event_record(ev0, stream)
+    for i in range(reps):
+        run(i)
+    hip_rt.event_record(ev1, stream)
+    hip_rt.event_sync(ev1)
+    us = hip_rt.event_elapsed_ms(ev0, ev1) * 1000.0 / reps
+    nbytes = P.chain_bytes(W, H)
+    return {"kernel": "speedOfLightKernel (csrc/speed_of_light.hip)", "us": round(us, 3),
+            "GBps": round(nbytes / (us * 1e-6) / 1e9, 1), "algorithmic_bytes": nbytes, "passes": reps}
+
+
+PARITY_CONFIGS = (("1024x1024", 1024, 1024, 1), ("4096x4096", 4096, 4096, 2), ("4095x2047", 4095, 2047, 3),
+                  ("1920x1080", 1920, 1080, 4), ("2047x4097", 2047, 4097, 5))
+
+
+def parity_outputs(stream):
+    """One chain of each sRGBA8 parity configuration and the 8192^2 RGBA16F chain
+    in both arithmetic modes, downloaded (level 0 from the device generator)."""
+    from vk_compute_mipmaps_amd import hip_rt, pyramid as P
+
+    out = {}
+    for name, w, h, seed in PARITY_CONFIGS:
+        c = P.DeviceMipChain(w, h)
+        c.synthesize(seed, stream)
+        c.dispatch(stream)
+        hip_rt.stream_sync(stream)
+        out[name] = (w, h, c.download())
+        c.free()
+    for arith, name in ((P.ARITH_FP32, "fp32"), (P.ARITH_PACKED_FP16, "packed_fp16")):
+        c = P.DeviceMipChain(8192, 8192, P.FORMAT_RGBA16F)
+        c.synthesize(7, stream)
+        c.dispatch(stream, arithmetic=arith)
+        hip_rt.stream_sync(stream)
+        out["8192x8192 RGBA16F " + name] = (8192, 8192, c.download())
+        c.free()
+    return out
+
+
+def parity_stats(outputs):
+    """The GPU chains against the oracle (CPU-baseline leg, outside every timed
+    region): for sRGBA8, the worst LSB and the bit-exact texel and RGB-channel
+    rates against the FP64 filter oracle, the worst LSB against the FP32
+    oracle, and the worst delta against the reference's CPU generator (the
+    `delta` of demo_app/mipmaps_app.cpp:825-833); for 8192^2 RGBA16F, the
+    per-level maximum relative RGB error against the FP64 chain."""
+    import numpy as np
+
+    from oracle import orc
+    from vk_compute_mipmaps_amd import pyramid as P
+
+    res = {}
+    for name, w, h, _ in PARITY_CONFIGS:
+        _, _, gpu = outputs[name]
+        host = np.zeros_like(gpu)
+        host[: w * h * 4] = gpu[: w * h * 4]
+        steps = orc.plan(w, h)
+        f32 = orc.pyramid_srgba8(host, w, h, steps, 0, "f32")
+        f64 = orc.pyramid_srgba8(host, w, h, steps, 0, "f64")
+        ref = orc.reference_cpu(host, w, h, threads=8)
+        d = np.abs(gpu[w * h * 4:].astype(np.int16) - f64[w * h * 4:].astype(np.int16)).reshape(-1, 4)
+        res[name + " sRGBA8"] = {
+            "fp32_oracle_worst_lsb": int(orc.compare(gpu, f32, w, h)[0]),
+            "fp64_worst_lsb": int(d.max()),
+            "fp64_texels_exact_pct": round(100.0 * float((d == 0).all(axis=1).mean()), 4),
+            "fp64_rgb_channels_exact_pct": round(100.0 * float((d[:, :3] == 0).mean()), 4),
+            "reference_cpu_delta": int(orc.compare(gpu, ref, w, h)[0]),
+            "texels": int(d.shape[0]),
+        }
+    w = h = 8192
+    offs, total = P.packed_layout(w, h, 0, 8)
+    halves = outputs["8192x8192 RGBA16F fp32"][2][: w * h * 8].view(np.float16)
+    chain = np.zeros(total // 2, np.float64)
+    chain[: halves.size] = halves.astype(np.float64)
+    f64 = orc.pyramid_linear_f64(chain, w, h)
+    for name in ("fp32", "packed_fp16"):
+        gpu = outputs["8192x8192 RGBA16F " + name][2].view(np.float16)
+        per = []
+        for level in range(1, len(offs)):
+            lw, lh = P.level_dims(w, h, level)
+            a, b = offs[level] // 2, offs[level] // 2 + lw * lh * 4
+            g = gpu[a:b].astype(np.float64).reshape(-1, 4)
+            r = f64[a:b].reshape(-1, 4)
+            per.append(float(f"{(np.abs(g[:, :3] - r[:, :3]) / r[:, :3]).max():.3e}"))
+        res["8192x8192 RGBA16F " + name] = {"max_rel_rgb_error_per_level": per}
+    return res
 
 
 def batched_leg():

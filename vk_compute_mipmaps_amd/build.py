@@ -21,6 +21,8 @@ LIB_PATH = os.path.join(LIB_DIR, "libnvpro_pyramid_hip.so")
 INCLUDE = os.path.join(REPO_DIR, "include")
 
 DEMO_PATH = os.path.join(LIB_DIR, "nvpro_pyramid_demo")
+# Measurement only: the speed-of-light pass the benchmark times beside the chain.
+SOL_PATH = os.path.join(LIB_DIR, "libnvpro_pyramid_sol.so")
 
 SOURCES = [
     "fast_pipeline.hip",
@@ -83,7 +85,19 @@ def build_library(verbose: bool = True) -> str:
     for obj in objs:
         os.remove(obj)
     build_demo(verbose)
+    build_sol(verbose)
     return LIB_PATH
+
+
+def build_sol(verbose: bool = True) -> str:
+    """The benchmark's speed-of-light kernel (csrc/speed_of_light.hip), its own library."""
+    tmp = SOL_PATH + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-shared", os.path.join(CSRC, "speed_of_light.hip"), "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, SOL_PATH)
+    return SOL_PATH
 
 
 def build_demo(verbose: bool = True) -> str:
