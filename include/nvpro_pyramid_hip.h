@@ -291,6 +291,27 @@ int nvproPyramidBatchAutotune(const NvproPyramidPipelines* pipelines, const Nvpr
 
 
 /* ------------------------------------------------------------------------- */
+/* Batched chains without allocation.  No counterpart in the reference.  For
+ * callers that cannot allocate device or pinned memory (code under hipGraph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): one batch in
+ * ROUNDS mode, planned and launched in one call, with exactly the launches
+ * nvproPyramidBatchCreate + nvproPyramidBatchLaunch make for that
+ * configuration (so the same bytes).  The work list goes to a static
+ * per-device scratch array of NVPRO_PYRAMID_NO_ALLOC_SCRATCH_BYTES (each GPU
+ * the library runs on holds it until the process ends) through one
+ * host-to-device copy from ordinary host memory on `stream`; a batch whose work
+ * list is larger is rejected with hipErrorInvalidValue before anything is
+ * enqueued (nvproPyramidGetLastError names the size).  Every call on a GPU
+ * shares that GPU's array, which keeps the last call's work list: calls that
+ * overlap in time on one GPU race, so order them on one stream.  config NULL:
+ * defaults with mode ROUNDS; mode must be ROUNDS; useGraph is ignored. */
+#define NVPRO_PYRAMID_NO_ALLOC_SCRATCH_BYTES (4u << 20)
+
+int nvproPyramidBatchDispatchNoAlloc(hipStream_t stream, const NvproPyramidPipelines* pipelines,
+                                     const NvproPyramidImage* images, const uint32_t* mipLevels,
+                                     uint32_t imageCount, const NvproPyramidBatchConfig* config);
+
+/* ------------------------------------------------------------------------- */
 /* Serving loop.  No reference counterpart (the reference regenerates one       */
 /* image's chain per frame, demo_app/mipmaps_app.cpp:362-369).  A server admits */
 /* mip-chain requests against an HBM budget, keeps each chain resident in its   */

@@ -33,6 +33,7 @@ SOURCES = [
     "convert.hip",
     "nvpro_pyramid_dispatch.cpp",
     "nvpro_pyramid_batch.cpp",
+    "nvpro_pyramid_noalloc.cpp",
     "nvpro_pyramid_server.cpp",
     "nvpro_pyramid_io.cpp",
 ]

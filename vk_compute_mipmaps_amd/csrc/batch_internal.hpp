@@ -2,6 +2,8 @@
 // serving loop: the work-list planner and a reusable device work list.
 #pragma once
 
+#include <stdint.h>
+
 #include <vector>
 
 #include "pyramid_internal.hpp"
@@ -50,11 +52,21 @@ public:
   // Grows the buffers to at least `bytes` now, so that work lists up to that
   // size never allocate (hipFree of an outgrown buffer waits for the device).
   hipError_t reserve(size_t bytes);
+  // Rounds mode without any allocation: the work list (which has no counters
+  // in rounds mode) goes to caller-owned device memory `dev` of at least
+  // roundsWorkListBytes(plan) bytes, through an ordinary host buffer `host`
+  // (grown as needed) and one host-to-device copy on `stream`.
+  static size_t roundsWorkListBytes(const BatchPlan& plan);
+  hipError_t stageRoundsInto(const BatchPlan& plan, const NvproPyramidPipelines& pipes,
+                             const NvproPyramidBatchConfig& cfg, hipStream_t stream, void* dev,
+                             std::vector<uint8_t>& host);
 
   uint32_t launches = 0, blocks = 0;
   size_t devBytes   = 0;
 
 private:
+  void bind(const BatchPlan& plan, const NvproPyramidPipelines& pipes, const NvproPyramidBatchConfig& cfg,
+            uint8_t* base, size_t oImg, size_t oSeg, size_t oItm, size_t oTl, size_t oCnt, size_t size);
   void* dev_ = nullptr;
   void* pinned_ = nullptr;
   void* extra_  = nullptr;

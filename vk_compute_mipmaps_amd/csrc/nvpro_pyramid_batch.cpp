@@ -194,46 +194,53 @@ BatchRunner::~BatchRunner()
     (void)hipHostFree(pinned_);
 }
 
-hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& pipes,
-                              const NvproPyramidBatchConfig& cfg, hipStream_t stream, const void* extra,
-                              size_t extraBytes)
+namespace {
+
+// Device layout of a work list: images | segments | items | tails | extra |
+// counters (zeroed; queue mode only).
+struct WorkListLayout
 {
-  // Device layout: images | segments | items | tails | extra | counters (zeroed).
-  const size_t oImg = 0;
-  const size_t oSeg = align256(oImg + p.imgs.size() * sizeof(ImageDesc));
-  const size_t oItm = align256(oSeg + p.segs.size() * sizeof(QueueSeg));
-  const size_t oTl  = align256(oItm + p.items.size() * sizeof(QueueItem));
-  const size_t oExt = align256(oTl + p.tails.size() * sizeof(TailPlan));
-  const size_t oCnt = align256(oExt + (extra ? extraBytes : 0u));
-  const size_t size = align256(oCnt + batchCounterWords(p.numSlots) * sizeof(uint32_t));
-  // Grow geometrically (the previous buffers are idle: the caller staged
-  // nothing in flight).  A serving loop stages a new work list per batch;
-  // growing to the exact size would reallocate often, and hipFree waits for
-  // the whole device.
-  if(size > devCap_ || size > pinnedCap_)
-    if(hipError_t e = reserve(std::max(size, 2u * std::min(devCap_, pinnedCap_))))
-      return e;
-  uint8_t* h = (uint8_t*)pinned_;
-  memset(h, 0, oCnt);
-  memcpy(h + oImg, p.imgs.data(), p.imgs.size() * sizeof(ImageDesc));
+  size_t oImg, oSeg, oItm, oTl, oExt, oCnt, size;
+};
+
+WorkListLayout layoutOf(const BatchPlan& p, size_t extraBytes, bool counters)
+{
+  WorkListLayout l;
+  l.oImg = 0;
+  l.oSeg = align256(l.oImg + p.imgs.size() * sizeof(ImageDesc));
+  l.oItm = align256(l.oSeg + p.segs.size() * sizeof(QueueSeg));
+  l.oTl  = align256(l.oItm + p.items.size() * sizeof(QueueItem));
+  l.oExt = align256(l.oTl + p.tails.size() * sizeof(TailPlan));
+  l.oCnt = align256(l.oExt + extraBytes);
+  l.size = align256(l.oCnt + (counters ? batchCounterWords(p.numSlots) * sizeof(uint32_t) : 0u));
+  return l;
+}
+
+// The host image of everything before the counters.
+void writeWorkList(uint8_t* h, const BatchPlan& p, const WorkListLayout& l, const void* extra, size_t extraBytes)
+{
+  memset(h, 0, l.oCnt);
+  memcpy(h + l.oImg, p.imgs.data(), p.imgs.size() * sizeof(ImageDesc));
   if(!p.segs.empty())
-    memcpy(h + oSeg, p.segs.data(), p.segs.size() * sizeof(QueueSeg));
+    memcpy(h + l.oSeg, p.segs.data(), p.segs.size() * sizeof(QueueSeg));
   if(!p.items.empty())
-    memcpy(h + oItm, p.items.data(), p.items.size() * sizeof(QueueItem));
+    memcpy(h + l.oItm, p.items.data(), p.items.size() * sizeof(QueueItem));
   if(!p.tails.empty())
-    memcpy(h + oTl, p.tails.data(), p.tails.size() * sizeof(TailPlan));
+    memcpy(h + l.oTl, p.tails.data(), p.tails.size() * sizeof(TailPlan));
   if(extra && extraBytes)
-    memcpy(h + oExt, extra, extraBytes);
-  // The work list is copied, the counters after it are zeroed on the device.
-  if(hipError_t e = launchStageCopy(stream, dev_, pinned_, oCnt, size))
-    return e;
-  uint8_t* base   = (uint8_t*)dev_;
+    memcpy(h + l.oExt, extra, extraBytes);
+}
+
+}  // namespace
+
+void BatchRunner::bind(const BatchPlan& p, const NvproPyramidPipelines& pipes, const NvproPyramidBatchConfig& cfg,
+                       uint8_t* base, size_t oImg, size_t oSeg, size_t oItm, size_t oTl, size_t oCnt, size_t size)
+{
   args_.imgs      = (const ImageDesc*)(base + oImg);
   args_.segs      = (const QueueSeg*)(base + oSeg);
   args_.items     = (const QueueItem*)(base + oItm);
   args_.tails     = (const TailPlan*)(base + oTl);
   args_.counters  = (uint32_t*)(base + oCnt);
-  extra_          = extra ? base + oExt : nullptr;
   args_.numSlots  = p.numSlots;
   args_.spinLimit = cfg.spinLimit ? cfg.spinLimit : kSpinLimit;
   format_         = pipes.format;
@@ -255,6 +262,48 @@ hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& p
   if(queue_ && items_ != 0)
     launches = 1u;
   devBytes = size;
+}
+
+hipError_t BatchRunner::stage(const BatchPlan& p, const NvproPyramidPipelines& pipes,
+                              const NvproPyramidBatchConfig& cfg, hipStream_t stream, const void* extra,
+                              size_t extraBytes)
+{
+  const WorkListLayout l = layoutOf(p, extra ? extraBytes : 0u, true);
+  // Grow geometrically (the previous buffers are idle: the caller staged
+  // nothing in flight).  A serving loop stages a new work list per batch;
+  // growing to the exact size would reallocate often, and hipFree waits for
+  // the whole device.
+  if(l.size > devCap_ || l.size > pinnedCap_)
+    if(hipError_t e = reserve(std::max(l.size, 2u * std::min(devCap_, pinnedCap_))))
+      return e;
+  writeWorkList((uint8_t*)pinned_, p, l, extra, extraBytes);
+  // The work list is copied, the counters after it are zeroed on the device.
+  if(hipError_t e = launchStageCopy(stream, dev_, pinned_, l.oCnt, l.size))
+    return e;
+  uint8_t* base = (uint8_t*)dev_;
+  extra_        = extra ? base + l.oExt : nullptr;
+  bind(p, pipes, cfg, base, l.oImg, l.oSeg, l.oItm, l.oTl, l.oCnt, l.size);
+  return hipSuccess;
+}
+
+size_t BatchRunner::roundsWorkListBytes(const BatchPlan& p) { return layoutOf(p, 0u, false).size; }
+
+hipError_t BatchRunner::stageRoundsInto(const BatchPlan& p, const NvproPyramidPipelines& pipes,
+                                        const NvproPyramidBatchConfig& cfg, hipStream_t stream, void* dev,
+                                        std::vector<uint8_t>& host)
+{
+  if(cfg.mode != NVPRO_PYRAMID_BATCH_MODE_ROUNDS)
+    return hipErrorInvalidValue;
+  const WorkListLayout l = layoutOf(p, 0u, false);
+  if(host.size() < l.size)
+    host.resize(l.size);
+  writeWorkList(host.data(), p, l, nullptr, 0u);
+  // A copy from ordinary (pageable) host memory: the runtime has taken the
+  // bytes when the call returns, so `host` may be reused at once.
+  if(hipError_t e = hipMemcpyAsync(dev, host.data(), l.oCnt, hipMemcpyHostToDevice, stream))
+    return e;
+  extra_ = nullptr;
+  bind(p, pipes, cfg, (uint8_t*)dev, l.oImg, l.oSeg, l.oItm, l.oTl, l.oCnt, l.size);
   return hipSuccess;
 }
 

@@ -53,6 +53,7 @@ EXPORTED_SYMBOLS = (
     "nvproPyramidBatchCreate",
     "nvproPyramidBatchLaunch",
     "nvproPyramidBatchGetInfo",
+    "nvproPyramidBatchDispatchNoAlloc",
     "nvproPyramidBatchReadErrors",
     "nvproPyramidBatchDestroy",
     "nvproPyramidBatchAutotune",
@@ -211,6 +212,9 @@ def lib():
                 ("nvproPyramidBatchGetInfo", [vp, ctypes.POINTER(NvproPyramidBatchInfo)]),
                 ("nvproPyramidBatchReadErrors", [vp, ctypes.POINTER(u32)]),
                 ("nvproPyramidBatchDestroy", [vp]),
+                ("nvproPyramidBatchDispatchNoAlloc", [vp, ctypes.POINTER(NvproPyramidPipelines),
+                                                      ctypes.POINTER(NvproPyramidImage), ctypes.POINTER(u32), u32,
+                                                      cfgp]),
                 ("nvproPyramidBatchAutotune", [ctypes.POINTER(NvproPyramidPipelines),
                                                ctypes.POINTER(NvproPyramidImage), ctypes.POINTER(u32), u32, vp, u32,
                                                cfgp, ctypes.POINTER(ctypes.c_float)])):
@@ -501,6 +505,18 @@ class PyramidBatch:
             self.destroy()
         except Exception:
             pass
+
+
+NO_ALLOC_SCRATCH_BYTES = 4 << 20  # NVPRO_PYRAMID_NO_ALLOC_SCRATCH_BYTES
+
+
+def batch_dispatch_no_alloc(stream: int, pipes: NvproPyramidPipelines, images, mip_levels=None,
+                            config: NvproPyramidBatchConfig | None = None):
+    """nvproPyramidBatchDispatchNoAlloc: one rounds-mode batch through the static scratch."""
+    arr = _image_array(images)
+    _check(lib().nvproPyramidBatchDispatchNoAlloc(ctypes.c_void_p(stream), ctypes.byref(pipes), arr,
+                                                  _mip_array(mip_levels, len(images)), len(images),
+                                                  ctypes.byref(config) if config is not None else None))
 
 
 def batch_autotune(pipes: NvproPyramidPipelines, images, mip_levels=None, stream: int = 0, repetitions: int = 5):
