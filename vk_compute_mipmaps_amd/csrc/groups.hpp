@@ -22,14 +22,20 @@ constexpr int kGroupBS = 256;  // threads per workgroup, every pipeline
 // lane owns a 4x4 input block (2x2 when COUNT is 1), loads it with one 16-byte
 // load per row and reduces it in registers to levels 1 and 2, so a group of 256
 // threads covers 4096 input texels: one 64x64 tile for COUNT 6, four 32x32
-// tiles for COUNT 5, and so on.  Lanes are numbered in Morton order within a
-// tile, so the level-2 values of a 2x2 are those of four consecutive threads.
+// tiles for COUNT 5, and so on.  The group's tiles are consecutive in tile
+// order (side by side along a tile row, except where a row ends), and lanes are
+// numbered row-major over the blocks of that strip (STRIP, the per-step
+// kernels' COUNT 6 groups), so the lanes of one wave load four texel rows of
+// 256 contiguous bytes per instruction instead of the eight 128-byte pieces of
+// a Morton order within the tile (the other counts and the batched kernel:
+// measured faster there).
 // Levels 3..COUNT are reduced by one "duty" wave per group (rotating over the
 // workgroup's waves with `iter`): the level-2 values go through a 4 KiB LDS
-// tile, duty lane k reduces threads 4k..4k+3 to level 3 (all 64 lanes busy), and
-// the levels above are cross-lane reductions inside the duty wave (DPP row
-// shifts, then ds_bpermute).  Per level the 2x2 summation order is the
-// reference's (fastOrder).
+// strip, duty lane k reads the 2x2 of level 2 under its level-3 texel (all 64
+// lanes busy; duty lanes are in Morton order within a tile), and the levels
+// above are cross-lane reductions inside the duty wave (DPP row shifts, then
+// ds_bpermute).  Per level the 2x2 summation order is the reference's
+// (fastOrder).
 // ---------------------------------------------------------------------------
 template <int COUNT>
 struct FastGeom
@@ -38,7 +44,39 @@ struct FastGeom
   static constexpr int TEAM         = COUNT >= 2 ? 2 * (COUNT - 2) : 0;  // threads per tile, log2 (<= 8)
   static constexpr int TILES_PER_WG = kGroupBS >> TEAM;
   static constexpr bool UPPER       = COUNT >= 3;                        // levels 3.. exist
+  static constexpr int T            = COUNT >= 2 ? 1 << (COUNT - 2) : 1;  // tile edge in lane blocks
+  static constexpr int RW           = TILES_PER_WG * T;                  // strip width in lane blocks
 };
+
+// Position of lane t in the group: its tile among the group's tiles and its
+// block (px, py) in that tile.  STRIP: row-major over the strip of the group's
+// tiles; otherwise Morton order within each tile, tiles one after the other.
+// Both layouts compute every value from the same operands in the same order.
+// Measured on MI355X (sRGBA8, per step): the strip takes F(0,6) of 16384^2
+// from 299 to 291 us and of 8192^2 RGBA16F from 142.5 to 137.9 us, but F(0,4)
+// of 3840x2160 from 10.2 to 10.7 us, F(0,5) of 2560x1440 from 6.6 to 6.8 us and
+// the one-group F(6,4) of 1024^2 by 0.9 us (one small tile then spans all four
+// waves); in the batched kernel its index arithmetic costs registers that the
+// many inlined bodies cannot spare.
+template <int COUNT, bool STRIP>
+__device__ __forceinline__ void fastLanePos(uint32_t t, uint32_t& tileInGroup, uint32_t& px, uint32_t& py)
+{
+  using G = FastGeom<COUNT>;
+  if constexpr(STRIP)
+  {
+    const uint32_t rx = t & (uint32_t)(G::RW - 1);
+    tileInGroup       = rx / (uint32_t)G::T;
+    px                = rx & (uint32_t)(G::T - 1);
+    py                = t / (uint32_t)G::RW;
+  }
+  else
+  {
+    const uint32_t inTile = t & ((1u << G::TEAM) - 1u);
+    tileInGroup           = t >> G::TEAM;
+    px                    = compactEven(inTile);
+    py                    = compactEven(inTile >> 1);
+  }
+}
 
 // Groups of a fast step over a width x height input level.
 __host__ __device__ inline uint32_t fastGroupCount(uint32_t width, uint32_t height, uint32_t count)
@@ -57,32 +95,42 @@ struct FastWindow
   typename F::Raw raw[1 << FastGeom<COUNT>::LB][1 << FastGeom<COUNT>::LB];
 };
 
-template <class F, int COUNT>
+// NT: load with the non-temporal hint (data read once).
+template <class F, int COUNT, bool STRIP, bool NT = false>
 __device__ __forceinline__ void fastLoad(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t numTiles,
                                          uint32_t g, FastWindow<F, COUNT>& w)
 {
   using G           = FastGeom<COUNT>;
   constexpr int LB  = G::LB;
   constexpr int BLK = 1 << LB;
-  const uint32_t tile = g * G::TILES_PER_WG + (threadIdx.x >> G::TEAM);
+  uint32_t tig, px, py;
+  fastLanePos<COUNT, STRIP>(threadIdx.x, tig, px, py);
+  const uint32_t tile = g * G::TILES_PER_WG + tig;
   if(tile >= numTiles)
     return;
-  const uint32_t inTile = threadIdx.x & ((1u << G::TEAM) - 1u);
-  const uint32_t px = compactEven(inTile), py = compactEven(inTile >> 1);
   const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
   const uint32_t x0 = (tx << COUNT) + (px << LB), y0 = (ty << COUNT) + (py << LB);
   const LevelDesc& in = img.lv[inLevel];
   const uint8_t* p    = img.base + in.offset + (uint64_t)y0 * in.pitch + (uint64_t)x0 * F::kBytes;
+  if constexpr(NT)
+  {
 #pragma unroll
-  for(int r = 0; r < BLK; ++r)
-    loadRow<F, BLK>(p + (uint64_t)r * in.pitch, w.raw[r]);
+    for(int r = 0; r < BLK; ++r)
+      loadRowStream<F, BLK>(p + (uint64_t)r * in.pitch, w.raw[r]);
+  }
+  else
+  {
+#pragma unroll
+    for(int r = 0; r < BLK; ++r)
+      loadRow<F, BLK>(p + (uint64_t)r * in.pitch, w.raw[r]);
+  }
 }
 
 // Group g of a fast step of `img` from level inLevel (tilesX x tilesY tiles in
 // all, numTiles = their product), from the lane blocks fastLoad put in `w`.
 // `level2`: kGroupBS values of LDS scratch (unused when COUNT < 3).  Contains
 // barriers when COUNT >= 3.
-template <class F, int COUNT, bool CO = false, class TT>
+template <class F, int COUNT, bool CO, bool STRIP, class TT>
 __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* level2, const ImageDesc& img,
                                             uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
                                             uint32_t iter, const FastWindow<F, COUNT>& w)
@@ -94,14 +142,13 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
   constexpr int BLK  = 1 << LB;
   constexpr bool PRE = F::kPrescale;
 
-  const uint32_t lane   = threadIdx.x & 63u;
-  const uint32_t wave   = threadIdx.x >> 6;
-  const uint32_t inTile = threadIdx.x & ((1u << G::TEAM) - 1u);
-  const uint32_t px     = compactEven(inTile);
-  const uint32_t py     = compactEven(inTile >> 1);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = threadIdx.x >> 6;
+  uint32_t tig, px, py;
+  fastLanePos<COUNT, STRIP>(threadIdx.x, tig, px, py);
 
   // Levels 1 and 2 of this thread's block.
-  const uint32_t tile = g * G::TILES_PER_WG + (threadIdx.x >> G::TEAM);
+  const uint32_t tile = g * G::TILES_PER_WG + tig;
   V v{};
   if(tile < numTiles)  // whole tiles
   {
@@ -143,13 +190,16 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
     __syncthreads();
     if(wave == (iter & 3u))
     {
-      // Duty lane k: level 3 from threads 4k..4k+3 (a Morton 2x2 of level 2).
+      // Duty lane k: level-3 texel (qx, qy) of tile lane >> TEAM3 (Morton
+      // order within the tile), from the 2x2 of level 2 under it: threads
+      // 4k..4k+3 in Morton order, or two rows of the strip.
       constexpr int TEAM3 = 2 * (COUNT - 3);  // duty lanes per tile, log2 (<= 6)
       const uint32_t t3   = g * G::TILES_PER_WG + (lane >> TEAM3);
       const uint32_t k3   = lane & ((1u << TEAM3) - 1u);
       const uint32_t qx   = compactEven(k3), qy = compactEven(k3 >> 1);
-      const V* c          = &level2[4u * lane];
-      V u                 = box<F, false>(c[0], c[1], c[2], c[3], fastOrder(COUNT, 3u));
+      const uint32_t b  = STRIP ? 2u * qy * (uint32_t)G::RW + (lane >> TEAM3) * (uint32_t)G::T + 2u * qx : 4u * lane;
+      const uint32_t dy = STRIP ? (uint32_t)G::RW : 2u;
+      V u = box<F, false>(level2[b], level2[b + 1u], level2[b + dy], level2[b + dy + 1u], fastOrder(COUNT, 3u));
       if(t3 < numTiles)
       {
         const uint32_t ty = t3 / tilesX, tx = t3 - ty * tilesX;
@@ -186,15 +236,16 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
 
 // Group g, loads included.  Contains barriers when COUNT >= 3.  CO: the last
 // level is stored with storeTexelCoherent (for a continuation in the same
-// launch, see arriveLast).
-template <class F, int COUNT, bool CO = false, class TT>
+// launch, see arriveLast).  STRIP: the lane layout (fastLanePos).  NT: the
+// input loads carry the non-temporal hint.
+template <class F, int COUNT, bool CO = false, bool STRIP = (COUNT == 6), bool NT = false, class TT>
 __device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2, const ImageDesc& img,
                                           uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
                                           uint32_t iter)
 {
   FastWindow<F, COUNT> w;
-  fastLoad<F, COUNT>(img, inLevel, tilesX, numTiles, g, w);
-  fastCompute<F, COUNT, CO>(tab, level2, img, inLevel, tilesX, numTiles, g, iter, w);
+  fastLoad<F, COUNT, STRIP, NT>(img, inLevel, tilesX, numTiles, g, w);
+  fastCompute<F, COUNT, CO, STRIP>(tab, level2, img, inLevel, tilesX, numTiles, g, iter, w);
 }
 
 // ---------------------------------------------------------------------------

@@ -74,6 +74,28 @@ __device__ __forceinline__ void loadRow(const uint8_t* p, typename F::Raw (&out)
   __builtin_memcpy(out, w, sizeof w);
 }
 
+// loadRow for data read once: 16-byte loads with the non-temporal hint (rows
+// whose size is not a multiple of 16 bytes load as loadRow does).
+template <class F, int N>
+__device__ __forceinline__ void loadRowStream(const uint8_t* p, typename F::Raw (&out)[N])
+{
+  constexpr int NBYTES = N * F::kBytes;
+  if constexpr(NBYTES % 16 == 0)
+  {
+    typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+    uint32_t w[NBYTES / 4];
+#pragma unroll
+    for(int i = 0; i < NBYTES / 16; ++i)
+    {
+      const U4 v = __builtin_nontemporal_load(reinterpret_cast<const U4*>(p) + i);
+      w[4 * i] = v.x, w[4 * i + 1] = v.y, w[4 * i + 2] = v.z, w[4 * i + 3] = v.w;
+    }
+    __builtin_memcpy(out, w, sizeof w);
+  }
+  else
+    loadRow<F, N>(p, out);
+}
+
 template <class F, int N>
 __device__ __forceinline__ void storeRow(uint8_t* p, const typename F::Raw (&in)[N])
 {
