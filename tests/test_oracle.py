@@ -566,3 +566,26 @@ def test_rgba16f_emulators_track_fp64(w, h, orc):
         rel_b = np.max(np.abs(b - ref) / np.abs(ref))
         assert rel_a < 2e-3, (l, rel_a)  # one half-ulp rounding (4.9e-4) per stored level at most 2 deep
         assert rel_b < 2e-2, (l, rel_b)
+
+
+@pytest.mark.parametrize("w,h", [(1024, 1024), (1920, 1080), (4095, 2047), (257, 3), (1, 37)])
+def test_rgba32f_emulator_tracks_fp64(w, h, orc):
+    """Pins orc_pyramid_rgba32f_f32 (the RGBA32F checker) against the FP64 filter
+    oracle: with no quantisation between levels, each level adds at most a few
+    FP32 roundings (the 2x2 box: three adds and an exact scale; the 3-tap: three
+    products or FMAs, weights rounded once), so the relative error stays below
+    levels x 2^-22 on positive values."""
+    rng = np.random.default_rng(w ^ h)
+    l0 = rng.random((h, w, 4), dtype=np.float32) * np.float32(4.0)
+    offs, total = P.packed_layout(w, h, 0, 16)
+    chain = np.zeros(total // 4, np.float32)
+    chain[: l0.size] = l0.reshape(-1)
+    emu = orc.pyramid_rgba32f(chain, w, h, H.to_orc_steps(P.plan(w, h, fmt=P.FORMAT_RGBA32F)))
+    ref = np.zeros(total // 4, np.float64)
+    ref[: l0.size] = l0.reshape(-1)
+    ref = orc.pyramid_linear_f64(ref.reshape(-1, 4), w, h).reshape(-1)
+    start = w * h * 4
+    rel = np.abs(emu[start:].astype(np.float64) - ref[start:]) / ref[start:]
+    bound = P.full_level_count(w, h) * 2.0 ** -22
+    assert rel.max() <= bound, f"{w}x{h}: relative error {rel.max():.3e} above {bound:.3e}"
+    assert rel.max() > 0.0
