@@ -136,6 +136,37 @@ def test_failed_batch_is_reported(stream):
         srv.destroy()
 
 
+def test_failed_launch_completes_its_requests(stream, monkeypatch):
+    """A batch whose launch fails after its staging copy is enqueued (forced from
+    the host: NVPRO_PYRAMID_SERVER_TEST_FAIL_LAUNCHES, read at creation) is not
+    lost: the poll reports the failure, its requests complete with a nonzero
+    status once the enqueued work is done, release frees their bytes, and the
+    server serves later requests normally."""
+    monkeypatch.setenv("NVPRO_PYRAMID_SERVER_TEST_FAIL_LAUNCHES", "1")
+    srv = P.PyramidServer(P.server_config(max_in_flight=2))
+    monkeypatch.delenv("NVPRO_PYRAMID_SERVER_TEST_FAIL_LAUNCHES")
+    try:
+        sizes = [(300, 200), (257, 3), (64, 64)]
+        tickets = [srv.submit(P.FORMAT_SRGBA8, w, h, seed=80 + k) for k, (w, h) in enumerate(sizes)]
+        with pytest.raises(P.NvproPyramidError, match="batch launch"):
+            srv.poll()
+        done = _drain(srv, tickets)
+        assert all(c.status != 0 for c in done.values())
+        assert srv.stats().failedBatches == 1
+        for t in tickets:
+            srv.release(t)
+        assert srv.stats().residentBytes == 0
+        again = {srv.submit(P.FORMAT_SRGBA8, w, h, seed=90 + k): (k, w, h) for k, (w, h) in enumerate(sizes)}
+        for t, c in _drain(srv, list(again)).items():
+            k, w, h = again[t]
+            assert c.status == 0
+            assert np.array_equal(_download(c.image, c.bytes), _alone(P.FORMAT_SRGBA8, w, h, 90 + k, stream))
+            srv.release(t)
+        assert srv.stats().failedBatches == 1 and srv.stats().residentBytes == 0
+    finally:
+        srv.destroy()
+
+
 def test_poisson_mix_at_stated_workload(stream):
     """The serving workload: Poisson arrivals, sizes of the 512-texture
     distribution (each side log-uniform in [64, 4096], 40 % odd), a 70/30 mix of

@@ -18,6 +18,7 @@
 // Every chain is planned exactly as nvproPyramidDispatch plans it, so a result
 // does not depend on what it was batched with.
 #include <stddef.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -126,7 +127,8 @@ struct Slot
   uint32_t* errHost  = nullptr;  // pinned copy of the batch's error flags (queue mode writes it)
   BatchRunner runner;           // work list, and the level-0 generator's jobs and chunks
   std::vector<uint64_t> tickets;
-  bool busy = false;
+  bool busy      = false;
+  int failStatus = 0;  // nonzero: the batch could not be launched; its requests complete with this
 };
 
 }  // namespace
@@ -144,6 +146,9 @@ struct NvproPyramidServer
   uint64_t nextTicket = 1;
   NvproPyramidServerStats stats{};
   std::mutex mtx;
+  // Test hook: the next this-many batch launches fail after their staging copy
+  // is enqueued (NVPRO_PYRAMID_SERVER_TEST_FAIL_LAUNCHES at creation).
+  uint32_t testFailLaunches = 0;
 };
 
 static uint64_t chainBytes(const NvproPyramidRequest& r)
@@ -170,9 +175,13 @@ static int retire(NvproPyramidServer* s, Clock::time_point now)
     const hipError_t q = hipEventQuery(slot->done);
     if(q == hipErrorNotReady)
       continue;
-    const int status = q != hipSuccess ? (int)q : (*slot->errHost ? (int)hipErrorUnknown : 0);
-    if(*slot->errHost)
+    const int status = q != hipSuccess       ? (int)q
+                       : slot->failStatus    ? slot->failStatus
+                       : *slot->errHost      ? (int)hipErrorUnknown
+                                             : 0;
+    if(slot->failStatus || *slot->errHost)
       s->stats.failedBatches++;
+    slot->failStatus = 0;
     for(uint64_t t : slot->tickets)
     {
       Request& r = s->requests[t];
@@ -242,6 +251,23 @@ static int launch(NvproPyramidServer* s)
         keep.push_back(t);
     }
     s->admitted.swap(keep);
+    // From here on a failure must not lose the taken requests: they complete
+    // with the error (their chains stay resident until released) once the work
+    // already enqueued on the slot's stream, which may still read the slot's
+    // staging, has finished; retire() does that when the slot's event fires.
+    auto abandon = [&](int code, const char* what) -> int {
+      slot->tickets    = take;
+      slot->failStatus = code ? code : (int)hipErrorUnknown;
+      if(hipEventRecord(slot->done, slot->stream) != hipSuccess)
+      {
+        (void)hipStreamSynchronize(slot->stream);  // then the event's last record has fired
+        (void)hipGetLastError();
+      }
+      slot->busy = true;
+      s->stats.batches++;
+      s->stats.batchedImages += take.size();
+      return hostFail(code, "%s: %s: %s", who, what, hipGetErrorString((hipError_t)code));
+    };
     std::vector<NvproPyramidImage> images;
     std::vector<uint32_t> mips;
     std::vector<SynthJob> jobs;
@@ -256,7 +282,11 @@ static int launch(NvproPyramidServer* s)
       {
         const uint64_t n = (uint64_t)r.req.width * r.req.height * bpt;
         if(hipError_t he = hipMemcpyAsync(images.back().data, r.req.level0, n, hipMemcpyHostToDevice, slot->stream))
-          return hostFail(he, "%s: level 0 upload: %s", who, hipGetErrorString(he));
+        {
+          for(uint64_t u : take)
+            s->requests[u].state = State::Running;
+          return abandon(he, "level 0 upload");
+        }
       }
       else
       {
@@ -284,10 +314,15 @@ static int launch(NvproPyramidServer* s)
                                       0u};
     BatchPlan plan;
     if(int e = buildBatchPlan(&pipes, images.data(), mips.data(), (uint32_t)images.size(), s->cfg.batch, plan, who))
-      return e;
+      return abandon(e, "batch plan");
     *slot->errHost = 0;  // the slot is idle: no launch writes it now
     hipError_t e   = slot->runner.stage(plan, pipes, s->cfg.batch, slot->stream, chunks.empty() ? nullptr : table.data(),
                                         table.size());
+    if(e == hipSuccess && s->testFailLaunches)
+    {
+      --s->testFailLaunches;
+      e = hipErrorLaunchFailure;
+    }
     if(e == hipSuccess && !chunks.empty())
     {
       const uint8_t* dev = (const uint8_t*)slot->runner.extraDevice();
@@ -296,10 +331,10 @@ static int launch(NvproPyramidServer* s)
     }
     if(e == hipSuccess)
       e = slot->runner.launch(slot->stream);
-    if(e == hipSuccess)
-      e = hipEventRecord(slot->done, slot->stream);
     if(e != hipSuccess)
-      return hostFail(e, "%s: batch launch: %s", who, hipGetErrorString(e));
+      return abandon(e, "batch launch");
+    if(hipError_t re = hipEventRecord(slot->done, slot->stream))
+      return abandon(re, "completion event");
     slot->tickets = take;
     slot->busy    = true;
     s->stats.batches++;
@@ -437,6 +472,8 @@ int nvproPyramidServerCreate(const NvproPyramidServerConfig* config, NvproPyrami
   }
   if(e == hipSuccess)
     e = warmUp(s);
+  if(const char* t = getenv("NVPRO_PYRAMID_SERVER_TEST_FAIL_LAUNCHES"))
+    s->testFailLaunches = (uint32_t)strtoul(t, nullptr, 10);
   if(e != hipSuccess)
   {
     destroyServer(s);
