@@ -191,6 +191,15 @@ const char* nvproPyramidGetLastError(void);
  * shows whether a dispatch was fused). */
 uint32_t nvproPyramidLastLaunchCount(void);
 
+/* Fused dispatches (a step and its small successor in one launch) count
+ * arrivals in a per-stream block of a static device array with 64 blocks per
+ * device; a stream keeps its block, and streams beyond the 64th dispatch
+ * unfused (same bytes, one more launch).  A caller that creates and destroys
+ * streams calls this before destroying one: it waits for the stream's work and
+ * hands its block to later streams.  No reference counterpart (Vulkan command
+ * buffers carry no such state). */
+int nvproPyramidStreamRelease(hipStream_t stream);
+
 /* Workload generator (no reference counterpart; the reference's benchmark loads
  * image files instead, demo_app/mipmaps_app.cpp:556-879): writes the seeded
  * synthetic level 0 of `image` on `stream` (SRGBA8: smooth value noise with a
@@ -283,8 +292,9 @@ int nvproPyramidBatchDestroy(NvproPyramidBatch* batch);
 /* Project autotuner: times each candidate configuration (mode, workgroups per
  * CU, item size, tail size; graph on) over `repetitions` launches of these
  * images on `stream` and returns the fastest.  It writes the images' output
- * levels (with the same values every time).  Results are cached per device,
- * format and batch shape for the life of the process. */
+ * levels (with the same values every time).  Results are cached for the life
+ * of the process per device, format, arithmetic and exact list of image sizes
+ * and level counts. */
 int nvproPyramidBatchAutotune(const NvproPyramidPipelines* pipelines, const NvproPyramidImage* images,
                               const uint32_t* mipLevels, uint32_t imageCount, hipStream_t stream,
                               uint32_t repetitions, NvproPyramidBatchConfig* best, float* bestMicroseconds);

@@ -58,6 +58,7 @@ def _check(chain, stream, **kw):
         assert n_other == n_fused
     finally:
         hip_rt.stream_sync(other)
+        P.stream_release(other)
         hip_rt.stream_destroy(other)
     # The requested levels were all written.
     levels = kw.get("mip_levels", 0) or P.full_level_count(chain.width, chain.height)
@@ -145,6 +146,7 @@ def test_fused_concurrent_streams_srgba8(stream):
             assert np.array_equal(c.download(), want[k]), f"{sizes[k]}: concurrent fused dispatches differ"
     finally:
         hip_rt.stream_sync(other)
+        P.stream_release(other)
         hip_rt.stream_destroy(other)
         for c in chains:
             c.free()
@@ -215,6 +217,32 @@ def test_fused_more_streams_than_counter_blocks_srgba8(stream):
         assert counts.count(1) <= 2 * 64
     finally:
         for s in streams:
-            hip_rt.stream_sync(s)
+            P.stream_release(s)
             hip_rt.stream_destroy(s)
+        chain.free()
+
+
+def test_released_counter_blocks_are_reused_srgba8(stream):
+    """nvproPyramidStreamRelease: 100 streams one after another, each released
+    before it is destroyed, all fuse (the released blocks go to the next
+    streams), and every result is the separate-launch bytes."""
+    chain = P.DeviceMipChain(4096, 4096, P.FORMAT_SRGBA8)
+    try:
+        chain.synthesize(10, stream)
+        want, _ = _run(chain, stream, False)
+        for k in range(100):
+            s = hip_rt.stream_create()
+            try:
+                chain.poison(stream=s)
+                chain.dispatch(stream=s)
+                assert P.last_launch_count() == 1, f"stream {k} did not fuse"
+                hip_rt.stream_sync(s)
+                assert np.array_equal(chain.download(), want), f"stream {k} differs"
+            finally:
+                P.stream_release(s)
+                hip_rt.stream_destroy(s)
+        s = hip_rt.stream_create()  # a stream that never dispatched: nothing to release
+        P.stream_release(s)
+        hip_rt.stream_destroy(s)
+    finally:
         chain.free()

@@ -513,16 +513,21 @@ int nvproPyramidBatchAutotune(const NvproPyramidPipelines* pipelines, const Nvpr
     return hostFail(hipErrorInvalidValue, "%s: null output", who);
   if(!pipelines || !images || imageCount == 0)
     return hostFail(hipErrorInvalidValue, "%s: no images", who);
-  // Cache key: device, format, arithmetic, image count and total base texels.
+  // Cache key: device, format, arithmetic and the exact list of image sizes and
+  // level counts (two batches with the same count and texels but different
+  // shapes can want different configurations).
   int dev = 0;
   (void)hipGetDevice(&dev);
-  uint64_t texels = 0;
+  using Key = std::vector<uint64_t>;
+  Key key{(uint64_t)dev, pipelines->format, pipelines->arithmetic, imageCount};
+  key.reserve(4u + 2u * imageCount);
   for(uint32_t i = 0; i < imageCount; ++i)
-    texels += (uint64_t)images[i].width * images[i].height;
-  using Key = std::tuple<int, uint32_t, uint32_t, uint32_t, uint64_t>;
+  {
+    key.push_back((uint64_t)images[i].width << 32 | images[i].height);
+    key.push_back(mipLevels ? mipLevels[i] : 0u);
+  }
   static std::mutex mtx;
   static std::map<Key, std::pair<NvproPyramidBatchConfig, float>> cache;
-  const Key key{dev, pipelines->format, pipelines->arithmetic, imageCount, texels};
   {
     std::lock_guard<std::mutex> lock(mtx);
     auto it = cache.find(key);

@@ -14,6 +14,7 @@
 #include <map>
 #include <mutex>
 #include <utility>
+#include <vector>
 
 #include "pyramid_internal.hpp"
 
@@ -50,13 +51,19 @@ int fail(int code, const char* fmt, ...)
 // block (no fusion) while the stream is being captured: a graph may be
 // launched on any stream, beside eager launches on the capturing one, and
 // would share their block; nor for a stream of another device than the
-// current one, nor once the slots are taken, nor when NVPRO_PYRAMID_FUSE=0 is
-// set in the environment.
+// current one, nor while every slot is held, nor when NVPRO_PYRAMID_FUSE=0 is
+// set in the environment.  nvproPyramidStreamRelease hands a stream's block
+// back (after the stream's work) for later streams.
+std::mutex gSlotLock;
+std::map<std::pair<int, unsigned long long>, uint32_t> gSlots;  // by device, stream id
+std::map<int, uint32_t> gSlotsUsed;                              // slots ever taken, by device
+std::map<int, std::vector<uint32_t>> gSlotsFree;                 // released slots, by device
+
 uint32_t* fusedCounters(hipStream_t stream)
 {
-  static std::mutex lock;
-  static std::map<std::pair<int, unsigned long long>, uint32_t> slots;  // by current device, stream id
-  static std::map<int, uint32_t> used;                                    // slots taken, by device
+  std::mutex& lock = gSlotLock;
+  auto& slots      = gSlots;
+  auto& used       = gSlotsUsed;
   const char* env = getenv("NVPRO_PYRAMID_FUSE");
   if(env && env[0] == '0')
     return nullptr;
@@ -75,16 +82,22 @@ uint32_t* fusedCounters(hipStream_t stream)
   if(it != slots.end())
     return fusedCounterBlock(it->second);
   int streamDevice = 0;
-  if(used[device] >= kFusedSlots || hipStreamGetDevice(stream, &streamDevice) != hipSuccess ||
+  std::vector<uint32_t>& freed = gSlotsFree[device];
+  if((freed.empty() && used[device] >= kFusedSlots) || hipStreamGetDevice(stream, &streamDevice) != hipSuccess ||
      device != streamDevice)
   {
     (void)hipGetLastError();
     return nullptr;
   }
-  uint32_t* block = fusedCounterBlock(used[device]);
+  const uint32_t slot = freed.empty() ? used[device] : freed.back();
+  uint32_t* block     = fusedCounterBlock(slot);
   if(!block)
     return nullptr;
-  slots[key] = used[device]++;
+  if(freed.empty())
+    used[device]++;
+  else
+    freed.pop_back();
+  slots[key] = slot;
   return block;
 }
 
@@ -483,3 +496,29 @@ extern "C" int nvproPyramidDispatch(hipStream_t stream, const NvproPyramidPipeli
 }
 
 extern "C" uint32_t nvproPyramidLastLaunchCount(void) { return gLastLaunches; }
+
+extern "C" int nvproPyramidStreamRelease(hipStream_t stream)
+{
+  const char* who = "nvproPyramidStreamRelease";
+  unsigned long long id = 0;
+  int device            = 0;
+  if(hipError_t e = hipStreamGetId(stream, &id))
+    return fail(e, "%s: hipStreamGetId: %s", who, hipGetErrorString(e));
+  if(hipError_t e = hipStreamGetDevice(stream, &device))
+    return fail(e, "%s: hipStreamGetDevice: %s", who, hipGetErrorString(e));
+  std::lock_guard<std::mutex> guard(gSlotLock);
+  const auto it = gSlots.find(std::make_pair(device, id));
+  if(it == gSlots.end())
+  {
+    gLastError[0] = 0;
+    return 0;  // the stream never fused a launch
+  }
+  // Its last fused launch must be over before another stream may take the
+  // block; every completed fused launch leaves the block zeroed.
+  if(hipError_t e = hipStreamSynchronize(stream))
+    return fail(e, "%s: hipStreamSynchronize: %s", who, hipGetErrorString(e));
+  gSlotsFree[device].push_back(it->second);
+  gSlots.erase(it);
+  gLastError[0] = 0;
+  return 0;
+}

@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "nvproPyramidBytesPerTexel",
     "nvproPyramidGetLastError",
     "nvproPyramidLastLaunchCount",
+    "nvproPyramidStreamRelease",
     "nvproPyramidSynthesize",
     "nvproPyramidPremultiplyAlpha",
     "nvproPyramidWriteTga",
@@ -175,6 +176,8 @@ def lib():
         L.nvproPyramidGetLastError.argtypes = []
         L.nvproPyramidLastLaunchCount.restype = ctypes.c_uint32
         L.nvproPyramidLastLaunchCount.argtypes = []
+        L.nvproPyramidStreamRelease.restype = ctypes.c_int
+        L.nvproPyramidStreamRelease.argtypes = [vp]
         L.nvproPyramidPackedLayout.restype = ctypes.c_int
         L.nvproPyramidPackedLayout.argtypes = [u32, u32, u32, u32, ctypes.POINTER(ctypes.c_uint64),
                                                ctypes.POINTER(ctypes.c_uint64)]
@@ -237,6 +240,11 @@ def lib():
 
 def last_error() -> str:
     return lib().nvproPyramidGetLastError().decode()
+
+
+def stream_release(stream: int):
+    """nvproPyramidStreamRelease: hand the stream's fused-launch counter block back."""
+    _check(lib().nvproPyramidStreamRelease(ctypes.c_void_p(stream)))
 
 
 def last_launch_count() -> int:
