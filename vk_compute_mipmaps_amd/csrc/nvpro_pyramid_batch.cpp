@@ -64,7 +64,15 @@ int nvpro_pyramid::buildBatchPlan(const NvproPyramidPipelines* pipelines, const 
     return hostFail(hipErrorInvalidValue, "%s: no images", who);
   const uint32_t bpt       = pipelines ? nvproPyramidBytesPerTexel(pipelines->format) : 0u;
   const uint32_t itemBytes = cfg.itemBytes ? cfg.itemBytes : kDefaultItemBytes;
-  plan                     = BatchPlan{};
+  // Cleared, not replaced: a serving loop plans into the same BatchPlan batch
+  // after batch and keeps its vectors' capacity.
+  plan.imgs.clear();
+  plan.segs.clear();
+  plan.items.clear();
+  plan.tails.clear();
+  plan.roundEnd.clear();
+  plan.numSlots         = 0;
+  plan.algorithmicBytes = 0;
   plan.imgs.resize(count);
   // Per image: its steps as segments (before item cutting) in step order.
   struct Pending

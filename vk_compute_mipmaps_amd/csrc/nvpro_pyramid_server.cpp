@@ -42,63 +42,85 @@ double usSince(Clock::time_point a, Clock::time_point b)
   return std::chrono::duration<double, std::micro>(b - a).count();
 }
 
-// First-fit allocator over [0, size) with coalescing frees.
+// Best-fit allocator over [0, size) with coalescing frees.  Free ranges are
+// indexed by offset (to coalesce with neighbours) and by size (to find the
+// smallest that fits), so an allocation or a free costs O(log n) in the number
+// of free ranges.  (A first-fit scan of an offset-ordered list cost the serving
+// loop tens of milliseconds per poll once thousands of resident chains had
+// fragmented an 80 GB arena.)
 class Arena
 {
 public:
   void reset(uint64_t size)
   {
-    free_.clear();
+    byOffset_.clear();
+    bySize_.clear();
     if(size)
-      free_[0] = size;
+      insert(0, size);
     used_ = peak_ = 0;
   }
   bool alloc(uint64_t bytes, uint64_t* off)
   {
-    bytes = (bytes + 255u) & ~uint64_t(255u);
-    for(auto it = free_.begin(); it != free_.end(); ++it)
-    {
-      if(it->second >= bytes)
-      {
-        *off                 = it->first;
-        const uint64_t rest  = it->second - bytes;
-        const uint64_t start = it->first + bytes;
-        free_.erase(it);
-        if(rest)
-          free_[start] = rest;
-        used_ += bytes;
-        peak_ = used_ > peak_ ? used_ : peak_;
-        return true;
-      }
-    }
-    return false;
+    bytes   = (bytes + 255u) & ~uint64_t(255u);
+    auto it = bySize_.lower_bound(bytes);
+    if(it == bySize_.end())
+      return false;
+    const uint64_t size = it->first, start = it->second;
+    bySize_.erase(it);
+    byOffset_.erase(start);
+    if(size > bytes)
+      insert(start + bytes, size - bytes);
+    *off = start;
+    used_ += bytes;
+    peak_ = used_ > peak_ ? used_ : peak_;
+    return true;
   }
   void release(uint64_t off, uint64_t bytes)
   {
     bytes = (bytes + 255u) & ~uint64_t(255u);
     used_ -= bytes;
-    auto next = free_.lower_bound(off);
-    if(next != free_.end() && off + bytes == next->first)
+    auto next = byOffset_.lower_bound(off);
+    if(next != byOffset_.end() && off + bytes == next->first)
     {
       bytes += next->second;
-      next = free_.erase(next);
+      eraseSized(next->first, next->second);
+      next = byOffset_.erase(next);
     }
-    if(next != free_.begin())
+    if(next != byOffset_.begin())
     {
       auto prev = std::prev(next);
       if(prev->first + prev->second == off)
       {
-        prev->second += bytes;
-        return;
+        off = prev->first;
+        bytes += prev->second;
+        eraseSized(prev->first, prev->second);
+        byOffset_.erase(prev);
       }
     }
-    free_[off] = bytes;
+    insert(off, bytes);
   }
   uint64_t used() const { return used_; }
   uint64_t peak() const { return peak_; }
+  size_t freeRanges() const { return byOffset_.size(); }
 
 private:
-  std::map<uint64_t, uint64_t> free_;
+  void insert(uint64_t off, uint64_t size)
+  {
+    byOffset_[off] = size;
+    bySize_.emplace(size, off);
+  }
+  void eraseSized(uint64_t off, uint64_t size)
+  {
+    auto range = bySize_.equal_range(size);
+    for(auto it = range.first; it != range.second; ++it)
+      if(it->second == off)
+      {
+        bySize_.erase(it);
+        return;
+      }
+  }
+  std::map<uint64_t, uint64_t> byOffset_;     // offset -> size
+  std::multimap<uint64_t, uint64_t> bySize_;  // size -> offset
   uint64_t used_ = 0, peak_ = 0;
 };
 
@@ -146,6 +168,15 @@ struct NvproPyramidServer
   uint64_t nextTicket = 1;
   NvproPyramidServerStats stats{};
   std::mutex mtx;
+  // Per-batch host scratch, reused from batch to batch (a batch of a thousand
+  // images fills megabytes; new vectors each time cost page faults).
+  std::vector<uint64_t> take, keep;
+  std::vector<NvproPyramidImage> images;
+  std::vector<uint32_t> mips;
+  std::vector<SynthJob> jobs;
+  std::vector<SynthChunk> chunks;
+  std::vector<uint8_t> table;
+  BatchPlan plan;
   // Test hook: the next this-many batch launches fail after their staging copy
   // is enqueued (NVPRO_PYRAMID_SERVER_TEST_FAIL_LAUNCHES at creation).
   uint32_t testFailLaunches = 0;
@@ -242,7 +273,10 @@ static int launch(NvproPyramidServer* s)
     if(!slot)
       return 0;
     const uint32_t format = s->requests[s->admitted.front()].req.format;
-    std::vector<uint64_t> take, keep;
+    std::vector<uint64_t>& take = s->take;
+    std::vector<uint64_t>& keep = s->keep;
+    take.clear();
+    keep.clear();
     for(uint64_t t : s->admitted)
     {
       if(s->requests[t].req.format == format && take.size() < s->cfg.maxBatchImages)
@@ -268,11 +302,16 @@ static int launch(NvproPyramidServer* s)
       s->stats.batchedImages += take.size();
       return hostFail(code, "%s: %s: %s", who, what, hipGetErrorString((hipError_t)code));
     };
-    std::vector<NvproPyramidImage> images;
-    std::vector<uint32_t> mips;
-    std::vector<SynthJob> jobs;
-    std::vector<SynthChunk> chunks;
-    const uint32_t bpt = nvproPyramidBytesPerTexel(format);
+    std::vector<NvproPyramidImage>& images = s->images;
+    std::vector<uint32_t>& mips            = s->mips;
+    std::vector<SynthJob>& jobs            = s->jobs;
+    std::vector<SynthChunk>& chunks        = s->chunks;
+    images.clear();
+    mips.clear();
+    jobs.clear();
+    chunks.clear();
+    const uint32_t bpt   = nvproPyramidBytesPerTexel(format);
+    uint64_t synthTexels = 0;
     for(uint64_t t : take)
     {
       Request& r = s->requests[t];
@@ -290,19 +329,27 @@ static int launch(NvproPyramidServer* s)
       }
       else
       {
-        const uint32_t j = (uint32_t)jobs.size();
         jobs.push_back(SynthJob{(uint8_t*)images.back().data, r.req.seed, r.req.width, r.req.height,
                                 r.req.width * bpt, format});
-        const uint64_t n = (uint64_t)r.req.width * r.req.height;
-        for(uint64_t first = 0; first < n; first += kSynthChunk)
-          chunks.push_back(SynthChunk{j, 0u, first, n - first < kSynthChunk ? n - first : kSynthChunk});
+        synthTexels += (uint64_t)r.req.width * r.req.height;
       }
       r.state = State::Running;
     }
+    // Chunks of the level-0 generator: kSynthChunk texels, or larger when the
+    // batch alone gives the launch enough of them (about 8k), so the table
+    // stays small (a thousand mixed images in 4096-texel chunks are 200k
+    // entries, 5 MB, to write and copy per batch).  Chunks of up to 65,536
+    // texels keep the generator's lattice cache usable at any width.
+    uint64_t chunk = kSynthChunk;
+    while(chunk < 65536u && synthTexels / chunk > 8192u)
+      chunk *= 2u;
+    for(uint32_t j = 0; j < (uint32_t)jobs.size(); ++j)
+      appendSynthChunks(chunks, j, jobs[j].width, jobs[j].height, chunk);
     // The level-0 generator's table (jobs, then chunks) travels with the work
     // list in one staging copy; one launch then writes every synthetic level 0
     // of the batch.
-    std::vector<uint8_t> table(jobs.size() * sizeof(SynthJob) + chunks.size() * sizeof(SynthChunk));
+    std::vector<uint8_t>& table = s->table;
+    table.resize(jobs.size() * sizeof(SynthJob) + chunks.size() * sizeof(SynthChunk));
     if(!chunks.empty())
     {
       memcpy(table.data(), jobs.data(), jobs.size() * sizeof(SynthJob));
@@ -312,7 +359,7 @@ static int launch(NvproPyramidServer* s)
                                       format == NVPRO_PYRAMID_FORMAT_RGBA16F ? s->cfg.arithmetic16f
                                                                              : NVPRO_PYRAMID_ARITH_FP32,
                                       0u};
-    BatchPlan plan;
+    BatchPlan& plan = s->plan;
     if(int e = buildBatchPlan(&pipes, images.data(), mips.data(), (uint32_t)images.size(), s->cfg.batch, plan, who))
       return abandon(e, "batch plan");
     *slot->errHost = 0;  // the slot is idle: no launch writes it now
@@ -364,7 +411,7 @@ static hipError_t warmUp(NvproPyramidServer* s)
       {
         SynthJob job;
         SynthChunk chunk;
-      } table{SynthJob{(uint8_t*)s->arenaBase, 1u, w, h, w * bpt, format}, SynthChunk{0u, 0u, 0u, (uint64_t)w * h}};
+      } table{SynthJob{(uint8_t*)s->arenaBase, 1u, w, h, w * bpt, format}, SynthChunk{0u, 0u, 0u, h}};
       const NvproPyramidPipelines pipes{
           format, 1u, format == NVPRO_PYRAMID_FORMAT_RGBA16F ? s->cfg.arithmetic16f : NVPRO_PYRAMID_ARITH_FP32, 0u};
       const uint32_t mips = 0;
@@ -448,12 +495,14 @@ int nvproPyramidServerCreate(const NvproPyramidServerConfig* config, NvproPyrami
   NvproPyramidServer* s = new NvproPyramidServer;
   s->cfg                = cfg;
   hipError_t e          = hipMalloc(&s->arenaBase, cfg.budgetBytes);
-  // A work list with the level-0 generator's table takes about 3 KB per image
-  // of a mixed-size batch; sized now for a full batch of such images, so that
-  // the staging of ordinary batches never grows (growth frees, and hipFree
-  // waits for every batch in flight).
+  // A work list with the level-0 generator's table takes up to about 4 KB per
+  // image of a mixed-size batch (sizes to 4096, 64 KiB items; up to 6 KB with
+  // RGBA16F's larger items count); sized now for a full batch of such images,
+  // so that the staging of ordinary batches never grows (growth frees, and
+  // hipFree waits for every batch in flight: 8-16 ms stalls of the poll loop
+  // were measured with the staging sized at 4 KB per image).
   const size_t stageReserve =
-      std::min<size_t>(16u << 20, std::max<size_t>(256u << 10, (size_t)cfg.maxBatchImages * 4096u));
+      std::min<size_t>(32u << 20, std::max<size_t>(256u << 10, (size_t)cfg.maxBatchImages * 8192u));
   for(uint32_t i = 0; e == hipSuccess && i < cfg.maxInFlight; ++i)
   {
     Slot* slot = new Slot;

@@ -1,6 +1,8 @@
 // Internal declarations shared by the C-ABI host code and the HIP kernels.
 #pragma once
 
+#include <vector>
+
 #include <stdint.h>
 
 #include <hip/hip_runtime.h>
@@ -154,7 +156,8 @@ uint32_t batchMaxWorkgroupsPerCU(uint32_t format, uint32_t arithmetic, bool queu
 hipError_t launchStageCopy(hipStream_t stream, void* dst, const void* src, size_t copyBytes, size_t totalBytes);
 
 // Seeded synthetic level 0 (synth.hip): one image per job; a batched launch
-// writes chunks of kSynthChunk texels of many jobs.
+// writes chunks of many jobs.  A chunk is a strip of the columns [x0, x0 + 256)
+// (or to the edge) by `rows` rows from y0.
 struct SynthJob
 {
   uint8_t* base;  // level 0
@@ -163,11 +166,13 @@ struct SynthJob
 };
 struct SynthChunk
 {
-  uint32_t job, pad;
-  uint64_t first, count;  // texels [first, first + count), row-major
+  uint32_t job, x0, y0, rows;
 };
-constexpr uint64_t kSynthChunk = 4096;  // 16 texels per thread: enough workgroups for one 1 MP image
+constexpr uint64_t kSynthChunk = 4096;  // texels per chunk of one image: enough workgroups for 1 MP
 hipError_t launchSynthBatch(hipStream_t stream, const SynthJob* jobs, const SynthChunk* chunks, uint32_t numChunks);
+// Appends the chunks of job `job` (about texelsPerChunk texels each); returns how many.
+uint32_t appendSynthChunks(std::vector<SynthChunk>& chunks, uint32_t job, uint32_t width, uint32_t height,
+                           uint64_t texelsPerChunk);
 
 // Launch a validated plan: per-step kernels for the large steps, then one tail
 // launch for the small remainder.  With `arrive` (kArriveWords zeroed counters
