@@ -146,8 +146,9 @@ uint32_t nvproPyramidDefaultGeneralDispatcher(NvproPyramidCmd* cmd, const NvproP
  * own launch, in whichever workgroup finishes last.  Results are the same bytes
  * either way.  This fusion needs a 16.6 KB block of arrival counters per
  * stream, taken from a static device array of 64 blocks per device (nothing is
- * allocated; the first 64 streams of a device that fuse keep a block until the
- * process ends, later streams do not fuse); it is off while the stream is being
+ * allocated; a stream that fuses keeps its block until nvproPyramidStreamRelease
+ * or the end of the process, and while 64 streams of a device hold one, others
+ * do not fuse); it is off while the stream is being
  * captured into a graph, and when the environment variable NVPRO_PYRAMID_FUSE
  * is "0". */
 int nvproPyramidDispatch(hipStream_t stream, const NvproPyramidPipelines* pipelines,
