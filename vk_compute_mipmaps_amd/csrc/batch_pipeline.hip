@@ -68,12 +68,12 @@ __device__ __forceinline__ void runItem(BatchShared<F>& sh, const ImageDesc* __r
   {
     switch(seg.kind)
     {
-      case 1: fastGroup<F, 1, CO, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 2: fastGroup<F, 2, CO, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 3: fastGroup<F, 3, CO, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 4: fastGroup<F, 4, CO, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 5: fastGroup<F, 5, CO, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
-      case 6: fastGroup<F, 6, CO, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 1: fastGroup<F, 1, CO, false, false, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 2: fastGroup<F, 2, CO, false, false, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 3: fastGroup<F, 3, CO, false, false, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 4: fastGroup<F, 4, CO, false, false, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 5: fastGroup<F, 5, CO, false, false, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
+      case 6: fastGroup<F, 6, CO, false, false, false>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, seg.numTiles, g, iter); break;
       case kSegGeneral1: generalOneGroup<F, CO>(sh.tab, img, seg.inputLevel, g); break;
       case kSegGeneral2: generalTwoGroupAny<F, CO>(sh.tab, scratch, img, seg.inputLevel, seg.tilesX, g); break;
       case kSegTail:

@@ -126,11 +126,57 @@ __device__ __forceinline__ void fastLoad(const ImageDesc& img, uint32_t inLevel,
   }
 }
 
+// Row addressing of a fast step's output levels inLevel+1..inLevel+COUNT.
+// HOIST: read from the image descriptor once, before any arithmetic.  The
+// descriptor is kernel-argument memory read through the scalar cache; read at
+// each level's first store, every read that misses stalls the wave there,
+// which is what a one-group successor step meets on levels no other workgroup
+// has addressed.  The empty asm makes the values (scalar registers) ready at
+// this point, so their loads overlap the input loads in flight.  Without HOIST
+// (the batched kernel, whose many inlined bodies cannot spare the registers)
+// each store reads its level's descriptor.
+template <int COUNT, bool HOIST>
+struct FastOut
+{
+  uint8_t* base[COUNT];
+  uint32_t pitch[COUNT];
+
+  __device__ __forceinline__ FastOut(const ImageDesc& img, uint32_t inLevel)
+  {
+#pragma unroll
+    for(int r = 0; r < COUNT; ++r)
+    {
+      const LevelDesc& d = img.lv[inLevel + 1u + (uint32_t)r];
+      base[r]            = img.base + d.offset;
+      pitch[r]           = d.pitch;
+      __asm__ volatile("" ::"s"(base[r]), "s"(pitch[r]));
+    }
+  }
+  // Texel (x, y) of level inLevel + r, r in 1..COUNT.
+  __device__ __forceinline__ uint8_t* at(int r, uint32_t x, uint32_t y, uint32_t bytes) const
+  {
+    return base[r - 1] + (uint64_t)y * pitch[r - 1] + (uint64_t)x * bytes;
+  }
+};
+
+template <int COUNT>
+struct FastOut<COUNT, false>
+{
+  const ImageDesc& img;
+  uint32_t inLevel;
+
+  __device__ __forceinline__ FastOut(const ImageDesc& i, uint32_t l) : img(i), inLevel(l) {}
+  __device__ __forceinline__ uint8_t* at(int r, uint32_t x, uint32_t y, uint32_t bytes) const
+  {
+    return texelPtr(img, inLevel + (uint32_t)r, x, y, bytes);
+  }
+};
+
 // Group g of a fast step of `img` from level inLevel (tilesX x tilesY tiles in
 // all, numTiles = their product), from the lane blocks fastLoad put in `w`.
 // `level2`: kGroupBS values of LDS scratch (unused when COUNT < 3).  Contains
 // barriers when COUNT >= 3.
-template <class F, int COUNT, bool CO, bool STRIP, class TT>
+template <class F, int COUNT, bool CO, bool STRIP, bool HOIST, class TT>
 __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* level2, const ImageDesc& img,
                                             uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
                                             uint32_t iter, const FastWindow<F, COUNT>& w)
@@ -146,6 +192,7 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
   const uint32_t wave = threadIdx.x >> 6;
   uint32_t tig, px, py;
   fastLanePos<COUNT, STRIP>(threadIdx.x, tig, px, py);
+  const FastOut<COUNT, HOIST> out(img, inLevel);
 
   // Levels 1 and 2 of this thread's block.
   const uint32_t tile = g * G::TILES_PER_WG + tig;
@@ -172,15 +219,15 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
         row[j] = F::store(tab, l1[i][j]);
       }
       if constexpr(CO && COUNT == 1)
-        storeTexelCoherent<F>(texelPtr(img, inLevel + 1u, x0 >> 1, (y0 >> 1) + i, F::kBytes), row[0]);
+        storeTexelCoherent<F>(out.at(1, x0 >> 1, (y0 >> 1) + i, F::kBytes), row[0]);
       else
-        storeRow<F, BLK / 2>(texelPtr(img, inLevel + 1u, x0 >> 1, (y0 >> 1) + i, F::kBytes), row);
+        storeRow<F, BLK / 2>(out.at(1, x0 >> 1, (y0 >> 1) + i, F::kBytes), row);
     }
     v = l1[0][0];
     if constexpr(LB == 2)
     {
       v = box<F, false>(l1[0][0], l1[0][1], l1[1][0], l1[1][1], fastOrder(COUNT, 2u));
-      storeTexelTo<F, CO && COUNT == 2>(texelPtr(img, inLevel + 2u, x0 >> 2, y0 >> 2, F::kBytes), F::store(tab, v));
+      storeTexelTo<F, CO && COUNT == 2>(out.at(2, x0 >> 2, y0 >> 2, F::kBytes), F::store(tab, v));
     }
   }
   if constexpr(G::UPPER)
@@ -203,9 +250,8 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
       if(t3 < numTiles)
       {
         const uint32_t ty = t3 / tilesX, tx = t3 - ty * tilesX;
-        storeTexelTo<F, CO && COUNT == 3>(
-            texelPtr(img, inLevel + 3u, (tx << (COUNT - 3)) + qx, (ty << (COUNT - 3)) + qy, F::kBytes),
-            F::store(tab, u));
+        storeTexelTo<F, CO && COUNT == 3>(out.at(3, (tx << (COUNT - 3)) + qx, (ty << (COUNT - 3)) + qy, F::kBytes),
+                                          F::store(tab, u));
 #pragma unroll
         for(int k = 0; k < 3; ++k)
         {
@@ -221,8 +267,8 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
           if((k3 & ((4u << (2 * k)) - 1u)) == 0u)
           {
             const uint32_t sft = (uint32_t)(r - 3);
-            uint8_t* const p   = texelPtr(img, inLevel + (uint32_t)r, (tx << (COUNT - r)) + (qx >> sft),
-                                          (ty << (COUNT - r)) + (qy >> sft), F::kBytes);
+            uint8_t* const p   = out.at(r, (tx << (COUNT - r)) + (qx >> sft), (ty << (COUNT - r)) + (qy >> sft),
+                                        F::kBytes);
             if(CO && r == COUNT)
               storeTexelCoherent<F>(p, F::store(tab, u));
             else
@@ -237,15 +283,17 @@ __device__ __forceinline__ void fastCompute(const TT& tab, typename F::Val* leve
 // Group g, loads included.  Contains barriers when COUNT >= 3.  CO: the last
 // level is stored with storeTexelCoherent (for a continuation in the same
 // launch, see arriveLast).  STRIP: the lane layout (fastLanePos).  NT: the
-// input loads carry the non-temporal hint.
-template <class F, int COUNT, bool CO = false, bool STRIP = (COUNT == 6), bool NT = false, class TT>
+// input loads carry the non-temporal hint.  HOIST: output addressing read up
+// front (FastOut).
+template <class F, int COUNT, bool CO = false, bool STRIP = (COUNT == 6), bool NT = false, bool HOIST = true,
+          class TT>
 __device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2, const ImageDesc& img,
                                           uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, uint32_t g,
                                           uint32_t iter)
 {
   FastWindow<F, COUNT> w;
   fastLoad<F, COUNT, STRIP, NT>(img, inLevel, tilesX, numTiles, g, w);
-  fastCompute<F, COUNT, CO, STRIP>(tab, level2, img, inLevel, tilesX, numTiles, g, iter, w);
+  fastCompute<F, COUNT, CO, STRIP, HOIST>(tab, level2, img, inLevel, tilesX, numTiles, g, iter, w);
 }
 
 // ---------------------------------------------------------------------------
