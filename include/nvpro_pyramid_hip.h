@@ -196,9 +196,10 @@ uint32_t nvproPyramidLastLaunchCount(void);
  * arrivals in a per-stream block of a static device array with 64 blocks per
  * device; a stream keeps its block, and streams beyond the 64th dispatch
  * unfused (same bytes, one more launch).  A caller that creates and destroys
- * streams calls this before destroying one: it waits for the stream's work and
- * hands its block to later streams.  No reference counterpart (Vulkan command
- * buffers carry no such state). */
+ * streams calls this before destroying one, once it enqueues no more work on
+ * it: it waits for the stream's work and hands its block to later streams.
+ * Other threads may dispatch meanwhile.  No reference counterpart (Vulkan
+ * command buffers carry no such state). */
 int nvproPyramidStreamRelease(hipStream_t stream);
 
 /* Workload generator (no reference counterpart; the reference's benchmark loads

@@ -506,19 +506,20 @@ extern "C" int nvproPyramidStreamRelease(hipStream_t stream)
     return fail(e, "%s: hipStreamGetId: %s", who, hipGetErrorString(e));
   if(hipError_t e = hipStreamGetDevice(stream, &device))
     return fail(e, "%s: hipStreamGetDevice: %s", who, hipGetErrorString(e));
-  std::lock_guard<std::mutex> guard(gSlotLock);
-  const auto it = gSlots.find(std::make_pair(device, id));
-  if(it == gSlots.end())
-  {
-    gLastError[0] = 0;
-    return 0;  // the stream never fused a launch
-  }
-  // Its last fused launch must be over before another stream may take the
-  // block; every completed fused launch leaves the block zeroed.
+  // The stream's last fused launch must be over before another stream may
+  // take its block (every completed fused launch leaves the block zeroed).
+  // The wait happens outside the lock, which every dispatch takes: other
+  // threads keep dispatching meanwhile.  The caller enqueues no more work on
+  // the stream once it releases it.
   if(hipError_t e = hipStreamSynchronize(stream))
     return fail(e, "%s: hipStreamSynchronize: %s", who, hipGetErrorString(e));
-  gSlotsFree[device].push_back(it->second);
-  gSlots.erase(it);
+  std::lock_guard<std::mutex> guard(gSlotLock);
+  const auto it = gSlots.find(std::make_pair(device, id));
+  if(it != gSlots.end())  // else the stream never fused a launch
+  {
+    gSlotsFree[device].push_back(it->second);
+    gSlots.erase(it);
+  }
   gLastError[0] = 0;
   return 0;
 }
