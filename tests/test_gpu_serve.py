@@ -230,3 +230,72 @@ def test_poisson_mix_at_stated_workload(stream):
                     np.testing.assert_array_equal(got.view(np.uint16), emu, err_msg=f"request {k} ({w}x{h})")
     finally:
         srv.destroy()
+
+
+def test_submitters_and_pollers_on_threads(stream):
+    """The entry points are thread-safe: three threads submit requests while
+    two others poll (and release what they consume once it is checked), so
+    submits, polls, releases and stats calls interleave.  Every request
+    completes exactly once, and equals its alone result."""
+    import threading
+
+    per, submitters = 24, 3
+    sizes = synth.mixed_sizes(per * submitters, 77, hi=1024)
+    fmts = [P.FORMAT_RGBA16F if k % 4 == 1 else P.FORMAT_SRGBA8 for k in range(len(sizes))]
+    srv = P.PyramidServer(P.server_config(max_in_flight=4, max_batch_images=8,
+                                          batch=P.batch_config(mode=P.BATCH_MODE_ROUNDS, use_graph=False)))
+    lock = threading.Lock()
+    tickets, done, errors = {}, {}, []
+    submitted = threading.Event()
+
+    def submit(part):
+        try:
+            for k in range(part * per, (part + 1) * per):
+                w, h = sizes[k]
+                t = srv.submit(fmts[k], w, h, seed=7000 + k)
+                with lock:
+                    tickets[t] = k
+                if k % 5 == 0:
+                    srv.stats()
+        except Exception as e:
+            errors.append(f"submitter {part}: {e!r}")
+
+    def consume(who):
+        t_end = time.time() + 90
+        try:
+            while time.time() < t_end:
+                for c in srv.poll(8):
+                    with lock:
+                        assert c.ticket not in done, f"ticket {c.ticket} completed twice"
+                        done[c.ticket] = (c.status, _download(c.image, c.bytes))
+                    srv.release(c.ticket)
+                with lock:
+                    if submitted.is_set() and len(done) == len(tickets):
+                        return
+            errors.append(f"poller {who}: only {len(done)} of {len(tickets)} requests completed")
+        except Exception as e:
+            errors.append(f"poller {who}: {e!r}")
+
+    try:
+        subs = [threading.Thread(target=submit, args=(p,)) for p in range(submitters)]
+        polls = [threading.Thread(target=consume, args=(p,)) for p in range(2)]
+        for t in subs + polls:
+            t.start()
+        for t in subs:
+            t.join(timeout=120)
+        submitted.set()
+        for t in polls:
+            t.join(timeout=120)
+        assert not any(t.is_alive() for t in subs + polls), "a thread did not finish"
+        assert not errors, "; ".join(errors)
+        assert len(done) == len(tickets) == per * submitters
+        for t, (status, got) in done.items():
+            k = tickets[t]
+            w, h = sizes[k]
+            assert status == 0
+            want = _alone(fmts[k], w, h, 7000 + k, stream)
+            assert np.array_equal(got, want), f"request {k} ({w}x{h}, format {fmts[k]}) differs from its alone result"
+        st = srv.stats()
+        assert st.failedBatches == 0 and st.residentBytes == 0 and st.completed == per * submitters
+    finally:
+        srv.destroy()
