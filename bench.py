@@ -20,9 +20,11 @@ the host as a baseline.  Beside it:
             budget (lib/nvpro_pyramid_demo serve): chains/s, p50/p99 latency,
             the highest rate within a 2 ms p99 and the saturation rate, beside
             the plain loop's chains/s;
-  configs   single chains of the other configurations (1024^2; the odd sizes
+  configs   chains of the other configurations (1024^2; the odd sizes
             4095x2047, 1920x1080, 2047x4097 and 4095^2; 16384^2 sRGBA8; 8192^2
-            RGBA16F with FP32 and packed-FP16 arithmetic): us and GB/s.
+            RGBA16F with FP32 and packed-FP16 arithmetic), each over rotating
+            copies of its image: us and GB/s per chain back to back, and a
+            lone chain's latency between events.
 """
 from __future__ import annotations
 
@@ -214,7 +216,7 @@ def main():
         hip_rt.event_destroy(e)
     for c in chains:
         c.free()
-    configs = None if args.no_configs else large_configs(stream)
+    configs = None if args.no_configs else large_configs(stream, args.rotate_bytes)
     # GPU outputs of the parity configurations (compared with the oracle in the
     # CPU-baseline leg, outside every timed region).
     parity_inputs = None if args.no_cpu_baseline else parity_outputs(stream)
@@ -472,10 +474,17 @@ def serving_leg(loop_chains_per_s=None):
     return summary
 
 
-def large_configs(stream, reps=50, warmup=10):
-    """Single chains of the other configurations: median us and algorithmic GB/s
-    over `reps` back-to-back chains after `warmup` untimed ones (a short burst
-    from idle runs the large chains up to 10 % slower)."""
+def large_configs(stream, rotate_bytes, bursts=10, burst=10, singles=20, warmup=10):
+    """Single chains of the other configurations, each over enough copies of its
+    image to exceed `rotate_bytes` (at most 64), so that level 0 is not in the
+    caches when a chain starts.  `us`: the median over `bursts` runs of `burst`
+    back-to-back chains between two events, per chain (the headline's method);
+    `single_us`: the median of `singles` chains with an event between every
+    two, which adds the events' own cost (~2-3 us per chain) and shows a lone
+    chain's latency.  Both after `warmup` untimed chains (a short burst from
+    idle runs the large chains up to 10 % slower)."""
+    import numpy as np
+
     from vk_compute_mipmaps_amd import hip_rt, pyramid as P
 
     res = {}
@@ -488,24 +497,43 @@ def large_configs(stream, reps=50, warmup=10):
                                          ("8192x8192 RGBA16F fp32", 8192, 8192, P.FORMAT_RGBA16F, P.ARITH_FP32, 7),
                                          ("8192x8192 RGBA16F packed_fp16", 8192, 8192, P.FORMAT_RGBA16F,
                                           P.ARITH_PACKED_FP16, 7)):
-        c = P.DeviceMipChain(w, h, fmt)
-        c.synthesize(seed, stream)
+        first = P.DeviceMipChain(w, h, fmt)
+        n_buf = int(max(1, min(64, np.ceil(rotate_bytes / first.nbytes))))
+        chains = [first] + [P.DeviceMipChain(w, h, fmt) for _ in range(n_buf - 1)]
+        for c in chains:
+            c.synthesize(seed, stream)
+        k = 0
+
+        def dispatch():
+            nonlocal k
+            chains[k % n_buf].dispatch(stream, arithmetic=arith)
+            k += 1
+
         for _ in range(warmup):
-            c.dispatch(stream, arithmetic=arith)
+            dispatch()
         hip_rt.stream_sync(stream)
-        evs = [hip_rt.event_create() for _ in range(reps + 1)]
+        evs = [hip_rt.event_create() for _ in range(max(bursts, singles) + 1)]
         hip_rt.event_record(evs[0], stream)
-        for i in range(reps):
-            c.dispatch(stream, arithmetic=arith)
+        for i in range(bursts):
+            for _ in range(burst):
+                dispatch()
             hip_rt.event_record(evs[i + 1], stream)
-        hip_rt.event_sync(evs[-1])
-        per = sorted(hip_rt.event_elapsed_ms(evs[i], evs[i + 1]) * 1000.0 for i in range(reps))
+        hip_rt.event_sync(evs[bursts])
+        per = sorted(hip_rt.event_elapsed_ms(evs[i], evs[i + 1]) * 1000.0 / burst for i in range(bursts))
+        hip_rt.event_record(evs[0], stream)
+        for i in range(singles):
+            dispatch()
+            hip_rt.event_record(evs[i + 1], stream)
+        hip_rt.event_sync(evs[singles])
+        single = sorted(hip_rt.event_elapsed_ms(evs[i], evs[i + 1]) * 1000.0 for i in range(singles))
         for e in evs:
             hip_rt.event_destroy(e)
         us = statistics.median(per)
-        res[name] = {"us": round(us, 2), "GBps": round(c.nbytes / (us * 1e-6) / 1e9, 1),
-                     "algorithmic_bytes": c.nbytes, "plan": " ".join(map(repr, P.plan(w, h, fmt=fmt)))}
-        c.free()
+        res[name] = {"us": round(us, 2), "GBps": round(first.nbytes / (us * 1e-6) / 1e9, 1),
+                     "single_us": round(statistics.median(single), 2), "buffers": n_buf,
+                     "algorithmic_bytes": first.nbytes, "plan": " ".join(map(repr, P.plan(w, h, fmt=fmt)))}
+        for c in chains:
+            c.free()
     return res
 
 
