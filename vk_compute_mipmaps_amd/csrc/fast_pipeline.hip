@@ -65,8 +65,10 @@ __global__ void __launch_bounds__(kGroupBS) __attribute__((amdgpu_waves_per_eu(F
 fastPipelineKernel(ImageDesc img, uint32_t inLevel, uint32_t tilesX, uint32_t numTiles, NextStep next, TailPlan tail)
 {
   __shared__ FastShared<F, COUNT, CONT> sh;
+  const uint32_t lines = prefetchLevels(img);
   F::template fill<kGroupBS>(sh.tab, F::kPrescale ? 0.25f : 1.0f);
   __syncthreads();
+  levelsReady(lines);
   const uint32_t numGroups = (numTiles + FastGeom<COUNT>::TILES_PER_WG - 1u) / FastGeom<COUNT>::TILES_PER_WG;
   uint32_t iter            = 0;
   for(uint32_t g = blockIdx.x; g < numGroups; g += gridDim.x, ++iter)

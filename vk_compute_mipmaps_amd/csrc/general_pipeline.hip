@@ -16,8 +16,10 @@ template <class F>
 __global__ void __launch_bounds__(kGroupBS) generalOneLevelKernel(ImageDesc img, uint32_t inLevel, uint32_t groups)
 {
   __shared__ typename F::Tables tab;
+  const uint32_t lines = prefetchLevels(img);
   F::template fill<kGroupBS>(tab, 1.0f);
   __syncthreads();
+  levelsReady(lines);
   for(uint32_t g = blockIdx.x; g < groups; g += gridDim.x)
     generalOneGroup<F>(tab, img, inLevel, g);
 }
@@ -28,8 +30,10 @@ __global__ void __launch_bounds__(kGroupBS) generalTwoLevelKernel(ImageDesc img,
                                                                 uint32_t numTiles)
 {
   __shared__ GeneralShared<F> sh;
+  const uint32_t lines = prefetchLevels(img);
   F::template fill<kGroupBS>(sh.tab, 1.0f);
   __syncthreads();
+  levelsReady(lines);
   for(uint32_t tile = blockIdx.x; tile < numTiles; tile += gridDim.x)
     generalTwoGroup<F, false, KX, KY>(sh.tab, sh.mid, img, inLevel, tilesX, tile);
 }

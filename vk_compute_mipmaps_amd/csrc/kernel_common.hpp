@@ -119,6 +119,23 @@ __device__ __forceinline__ void storeTexel(uint8_t* p, typename F::Raw v)
   storeRow<F, 1>(p, t);
 }
 
+// Reads a word of every level descriptor, so that the scalar cache holds the
+// whole level table (kernel-argument memory) before a kernel addresses any
+// level: otherwise each level's first use that falls in a new 64-byte line
+// misses and stalls the wave there (about 0.3 us on MI355X), which a kernel
+// that walks many small levels pays once per line.  Call at kernel start and
+// pass the result to levelsReady() after the table fill's barrier, so that the
+// loads' latency overlaps the fill.
+__device__ __forceinline__ uint32_t prefetchLevels(const ImageDesc& img)
+{
+  uint32_t acc = img.levelCount;
+#pragma unroll
+  for(uint32_t k = 0; k < NVPRO_PYRAMID_MAX_LEVELS; ++k)
+    acc ^= img.lv[k].pitch;
+  return acc;
+}
+__device__ __forceinline__ void levelsReady(uint32_t acc) { __asm__ volatile("" ::"s"(acc)); }
+
 __device__ __forceinline__ uint8_t* texelPtr(const ImageDesc& img, uint32_t level, uint32_t x, uint32_t y,
                                              uint32_t bytes)
 {

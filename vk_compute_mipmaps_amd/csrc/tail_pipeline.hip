@@ -18,8 +18,13 @@ template <class F, int MAXT>
 __global__ void __launch_bounds__(kTailBS) tailKernel(ImageDesc img, TailPlan plan)
 {
   __shared__ TailShared<F, MAXT> sh;
+  uint32_t lines = prefetchLevels(img) ^ plan.count;  // the plan's lines too
+#pragma unroll
+  for(uint32_t k = 0; k < NVPRO_PYRAMID_MAX_LEVELS; ++k)
+    lines ^= plan.steps[k].inputLevel;
   F::template fill<kTailBS>(sh.tab, 1.0f);
   __syncthreads();
+  levelsReady(lines);
   tailRun<F, kTailBS>(sh.tab, sh.buf[0], sh.buf[1], img, plan);
 }
 
