@@ -6,8 +6,8 @@ the foreign calls, so the library really runs concurrently.  The entry points
 keep no per-call global state besides each stream's fused-launch counter block
 (taken under a lock, handed back by nvproPyramidStreamRelease) and the calling
 thread's error message and launch count, so every thread's bytes must equal a
-dispatch made alone.  The threads come in rounds; each round's streams are
-released and destroyed before the next round creates its own, so later rounds
+dispatch made alone.  The threads come in waves; each wave's streams are
+released and destroyed before the next wave creates its own, so later waves
 run on counter blocks that earlier streams used.  Each thread also launches a
 small batch of its own (rounds mode, no graph) between its dispatches.
 """
@@ -28,7 +28,7 @@ SIZES = [(4096, 4096, P.FORMAT_SRGBA8), (1024, 1024, P.FORMAT_SRGBA8), (4095, 20
 BATCH = [(64, 64), (257, 129), (1000, 750), (33, 1025)]
 SEED = 41
 THREADS = 6
-ROUNDS = 3
+WAVES = 3
 REPEATS = 4
 
 
@@ -103,7 +103,7 @@ def _worker(tid, start, want, errors):
 
 def test_threads_dispatch_concurrently_like_alone(stream):
     want = _alone(stream)
-    for r in range(ROUNDS):
+    for r in range(WAVES):
         errors = []
         start = threading.Barrier(THREADS, timeout=120)
         threads = [threading.Thread(target=_worker, args=(r * THREADS + t, start, want, errors))
@@ -112,5 +112,5 @@ def test_threads_dispatch_concurrently_like_alone(stream):
             t.start()
         for t in threads:
             t.join(timeout=300)
-            assert not t.is_alive(), f"round {r}: a thread did not finish"
-        assert not errors, f"round {r}: " + "; ".join(errors[:8])
+            assert not t.is_alive(), f"wave {r}: a thread did not finish"
+        assert not errors, f"wave {r}: " + "; ".join(errors[:8])
