@@ -145,21 +145,56 @@ def test_product_tables_match_oracle(orc):
     buckets = block("NVPRO_SRGB_ENCODE_BUCKETS")
     min_exp = int(re.search(r"NVPRO_SRGB_ENC_MIN_EXP (\d+)u", text).group(1))
     mant = int(re.search(r"NVPRO_SRGB_ENC_MANT_BITS (\d+)u", text).group(1))
-    n = len(buckets) // 2
-    for i in range(n):
-        t, lo = buckets[2 * i], buckets[2 * i + 1]
-        start = (min_exp << 23) + (i << (23 - mant))
-        end = start + (1 << (23 - mant))
+    assert mant == 7, "a bucket's floats differ in their low 16 bits"
+    # The buckets cover [2^-13, 1): the largest float below 1 is their last.
+    assert (min_exp << 23) + (len(buckets) << 16) == 0x3F800000
+    assert L.orc_srgb_encode_exact(bits_f32(0x3F7FFFFF)) == 255
+    assert L.orc_srgb_encode_exact(bits_f32((min_exp << 23) - 1)) == 0
+    for i, e in enumerate(buckets):
+        lo, s = e >> 16, e & 0xFFFF
+        assert e >> 24 == 0
+        start = (min_exp << 23) + (i << 16)
+        end = start + (1 << 16)
         assert lo == L.orc_srgb_encode_exact(bits_f32(start))
         steps = [k for k in range(1, 256) if start <= thr[k] < end]
         assert len(steps) <= 1
-        assert t == (thr[steps[0]] if steps else 0x7FC00000)
-        # Every float of the bucket encodes to lo or lo + 1 by the stored threshold.
-        for b in (start, end - 1, (start + end) // 2):
-            x = bits_f32(b)
-            want = L.orc_srgb_encode_exact(x)
-            got = lo + (1 if (t != 0x7FC00000 and b >= t) else 0)
+        assert s == ((thr[steps[0]] & 0xFFFF) - 1 if steps else 0xFFFF)
+        # Floats of the bucket encode to lo or lo + 1 by the stored step: its
+        # ends, its middle and both sides of the step.
+        probes = {start, end - 1, (start + end) // 2}
+        if steps:
+            probes |= {thr[steps[0]] - 1, thr[steps[0]]}
+        for b in probes:
+            want = L.orc_srgb_encode_exact(bits_f32(b))
+            got = lo + (1 if (b & 0xFFFF) > s else 0)
             assert got == want
+
+
+def test_encode_buckets_every_float(orc):
+    """Every float of [2^-13, 1) encodes through the kernels' bucket table (the
+    clamp, the bucket index, the low-16-bit step test of Srgba8::encodeChannel)
+    to the code the oracle's exact thresholds give: the number of thresholds at
+    or below it.  All 109M floats, in numpy."""
+    import re
+
+    text = open(P.PKG_DIR + "/csrc/srgb_tables.inc").read()
+    body = text.split("#define NVPRO_SRGB_ENCODE_BUCKETS \\")[1].split("\n\n")[0]
+    enc = np.array([int(t.rstrip("u"), 0) for t in re.findall(r"0x[0-9a-f]+u", body)], dtype=np.uint32)
+    min_exp = int(re.search(r"NVPRO_SRGB_ENC_MIN_EXP (\d+)u", text).group(1))
+    L = orc.lib()
+    thr = np.array([f32_bits(L.orc_srgb_threshold(k)) for k in range(1, 256)], dtype=np.uint32)
+    lo, hi = min_exp << 23, 0x3F800000
+    chunk = 1 << 23
+    for start in range(lo, hi, chunk):
+        b = np.arange(start, min(start + chunk, hi), dtype=np.uint32)
+        want = np.searchsorted(thr, b, side="right")
+        e = enc[(b >> 16) - (min_exp << 7)]
+        got = (e >> 16) + ((b & 0xFFFF) > (e & 0xFFFF))
+        bad = np.flatnonzero(got != want)
+        assert bad.size == 0, f"float bits 0x{int(b[bad[0]]):08x}: table {int(got[bad[0]])}, exact {int(want[bad[0]])}"
+    # The clamp's ends: below 2^-13 (and NaN, negatives) the code is 0, from 1 up 255.
+    assert np.searchsorted(thr, np.uint32(lo - 1), side="right") == 0
+    assert np.searchsorted(thr, np.uint32(hi - 1), side="right") == 255
 
 
 # ---------------------------------------------------------------------------

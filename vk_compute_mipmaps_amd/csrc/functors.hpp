@@ -32,8 +32,8 @@
 
 namespace nvpro_pyramid {
 
-static __constant__ uint32_t gSrgbDecode[256]                        = {NVPRO_SRGB_DECODE_BITS};
-static __constant__ uint32_t gSrgbEncode[2 * NVPRO_SRGB_ENC_BUCKETS] = {NVPRO_SRGB_ENCODE_BUCKETS};
+static __constant__ __attribute__((aligned(16))) uint32_t gSrgbDecode[256]                    = {NVPRO_SRGB_DECODE_BITS};
+static __constant__ __attribute__((aligned(16))) uint32_t gSrgbEncode[NVPRO_SRGB_ENC_BUCKETS] = {NVPRO_SRGB_ENCODE_BUCKETS};
 
 __device__ __forceinline__ float4 f4add(float4 a, float4 b)
 {
@@ -133,45 +133,39 @@ struct Srgba8 : Fp32Arith
   //
   // LDS tables: the decoded value of every RGB code (times the fill scale) and
   // the encode buckets.
-  struct Tables
+  struct __attribute__((aligned(16))) Tables
   {
     float dec[256];
-    uint2 enc[NVPRO_SRGB_ENC_BUCKETS];  // {first float with a higher code, code at start}
+    uint32_t enc[NVPRO_SRGB_ENC_BUCKETS];  // code at bucket start << 16 | low 16 bits of the step's float - 1
   };
+  static_assert(NVPRO_SRGB_ENC_BUCKETS % 4u == 0u, "16-byte table loads");
+  static_assert(23u - NVPRO_SRGB_ENC_MANT_BITS == 16u, "the floats of a bucket differ in their low 16 bits");
 
   template <int BS>
   __device__ static void fill(Tables& t, float scale)
   {
-    // Issue every global load before any LDS store.
-    constexpr int kEnc = (NVPRO_SRGB_ENC_BUCKETS + BS - 1) / BS;
-    constexpr int kDec = (256 + BS - 1) / BS;
-    uint2 e[kEnc];
-    float d[kDec];
+    // 16-byte loads, every global load issued before any LDS store.
+    constexpr int kEnc = (NVPRO_SRGB_ENC_BUCKETS / 4 + BS - 1) / BS;
+    static_assert(BS >= 64, "the decode table is one 16-byte load per thread of the first 64");
+    const uint4* genc = reinterpret_cast<const uint4*>(gSrgbEncode);
+    uint4 e[kEnc];
 #pragma unroll
     for(int k = 0; k < kEnc; ++k)
     {
       const uint32_t i = threadIdx.x + k * BS;
-      e[k] = i < NVPRO_SRGB_ENC_BUCKETS ? make_uint2(gSrgbEncode[2u * i], gSrgbEncode[2u * i + 1u]) : make_uint2(0, 0);
+      e[k] = i < NVPRO_SRGB_ENC_BUCKETS / 4u ? genc[i] : make_uint4(0, 0, 0, 0);
     }
-#pragma unroll
-    for(int k = 0; k < kDec; ++k)
-    {
-      const uint32_t i = threadIdx.x + k * BS;
-      d[k] = i < 256u ? __uint_as_float(gSrgbDecode[i]) : 0.0f;
-    }
-#pragma unroll
-    for(int k = 0; k < kDec; ++k)
-    {
-      const uint32_t i = threadIdx.x + k * BS;
-      if(i < 256u)
-        t.dec[i] = d[k] * scale;
-    }
+    const uint4 d = threadIdx.x < 64u ? reinterpret_cast<const uint4*>(gSrgbDecode)[threadIdx.x] : make_uint4(0, 0, 0, 0);
+    if(threadIdx.x < 64u)
+      reinterpret_cast<float4*>(t.dec)[threadIdx.x] =
+          make_float4(__uint_as_float(d.x) * scale, __uint_as_float(d.y) * scale, __uint_as_float(d.z) * scale,
+                      __uint_as_float(d.w) * scale);
 #pragma unroll
     for(int k = 0; k < kEnc; ++k)
     {
       const uint32_t i = threadIdx.x + k * BS;
-      if(i < NVPRO_SRGB_ENC_BUCKETS)
-        t.enc[i] = e[k];
+      if(i < NVPRO_SRGB_ENC_BUCKETS / 4u)
+        reinterpret_cast<uint4*>(t.enc)[i] = e[k];
     }
   }
 
@@ -207,15 +201,15 @@ struct Srgba8 : Fp32Arith
     return PRE ? f4mul(v, 0.25f) : v;
   }
 
-  // srgbFromLinear (preamble:110-121), exact: the bucket of x's top bits gives the
-  // code at the bucket start and the float at which it steps up.
+  // srgbFromLinear (preamble:110-121), exact: x is clamped to [2^-13, 1) (every
+  // smaller value, and NaN, encodes to 0, every larger one to 255); the bucket
+  // of its exponent and top mantissa bits holds the code at the bucket start
+  // and where in the bucket the code steps up.
   __device__ static uint32_t encodeChannel(const Tables& t, float x)
   {
-    float xc     = fminf(fmaxf(x, 0x1p-13f), 0x1.fffffep0f);
-    uint32_t idx = (__float_as_uint(xc) >> (23u - NVPRO_SRGB_ENC_MANT_BITS))
-                   - (NVPRO_SRGB_ENC_MIN_EXP << NVPRO_SRGB_ENC_MANT_BITS);
-    uint2 e = t.enc[idx];
-    return e.y + (x >= __uint_as_float(e.x) ? 1u : 0u);
+    const uint32_t b = __float_as_uint(fminf(fmaxf(x, 0x1p-13f), 0x1.fffffep-1f));
+    const uint32_t e = t.enc[(b >> (23u - NVPRO_SRGB_ENC_MANT_BITS)) - (NVPRO_SRGB_ENC_MIN_EXP << NVPRO_SRGB_ENC_MANT_BITS)];
+    return (e >> 16) + ((b & 0xffffu) > (e & 0xffffu) ? 1u : 0u);
   }
 
   // srgbFromLinearVec (preamble:123-128); alpha (code units) = floor(a + 0.5)
