@@ -17,9 +17,10 @@ the host as a baseline.  Beside it:
   serving   Poisson request streams at 20k to 200k requests/s (sizes of the
             mixed-size workload, 70% sRGBA8 / 30% RGBA16F, level 0 generated on
             the device per request) through the serving loop with an 80 GB
-            budget (lib/nvpro_pyramid_demo serve): chains/s, p50/p99 latency,
-            the highest rate within a 2 ms p99 and the saturation rate, beside
-            the plain loop's chains/s;
+            budget (lib/nvpro_pyramid_demo serve), each after a quarter second
+            of uncounted warm-up arrivals: chains/s, p50/p99 latency, the
+            highest rate within a 2 ms p99 and the saturation rate, beside the
+            plain loop's chains/s;
   configs   chains of the other configurations (1024^2; the odd sizes
             4095x2047, 1920x1080, 2047x4097 and 4095^2; 16384^2 sRGBA8; 8192^2
             RGBA16F with FP32 and packed-FP16 arithmetic), each over rotating
@@ -427,15 +428,18 @@ def batched_leg():
     return out
 
 
-SERVE_RATES = (20000, 40000, 80000, 120000, 160000, 200000)
+SERVE_RATES = (20000, 40000, 80000, 120000, 140000, 160000, 200000)
 SERVE_P99_BOUND_US = 2000.0
+SERVE_WARMUP_S = 0.25  # arrivals served before the measured ones at each rate, not counted
 
 
 def serving_leg(loop_chains_per_s=None):
     """Poisson request streams through the serving loop (C++ host), one run per
-    arrival rate (half a second of arrivals each): chains/s, p50/p99 latency and
-    batch sizes; `capacity` is the highest rate whose p99 stays within
-    SERVE_P99_BOUND_US, `saturation` the most chains/s any rate reached."""
+    arrival rate: SERVE_WARMUP_S seconds of arrivals that bring the GPU's clocks
+    and the server to a steady state and are not counted, then half a second of
+    measured arrivals: chains/s, p50/p99 latency and batch sizes; `capacity` is
+    the highest rate whose p99 stays within SERVE_P99_BOUND_US, `saturation` the
+    most chains/s any rate reached."""
     import subprocess
 
     from vk_compute_mipmaps_amd import build
@@ -443,7 +447,9 @@ def serving_leg(loop_chains_per_s=None):
     runs = []
     for rate in SERVE_RATES:
         count = max(10000, rate // 2)
-        cmd = [build.DEMO_PATH, "serve", "--rate", str(rate), "--count", str(count), "--seed", "9"]
+        warm = int(rate * SERVE_WARMUP_S)
+        cmd = [build.DEMO_PATH, "serve", "--rate", str(rate), "--count", str(count), "--warmup", str(warm),
+               "--seed", "9"]
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
         if r.returncode != 0:
             raise RuntimeError(f"{' '.join(cmd)} failed ({r.returncode}): {r.stdout[-2000:]} {r.stderr[-2000:]}")
@@ -461,7 +467,8 @@ def serving_leg(loop_chains_per_s=None):
         "saturation": {"rate_per_s": best["rate_per_s"], "chains_per_s": best["chains_per_s"],
                        "latency_us": best["latency_us"], "mean_batch_images": best["mean_batch_images"]},
         "plain_loop_chains_per_s": loop_chains_per_s,
-        "runs": [{k: o[k] for k in ("rate_per_s", "requests", "chains_per_s", "latency_us", "mean_batch_images",
+        "runs": [{k: o[k] for k in ("rate_per_s", "requests", "warmup_requests", "chains_per_s", "latency_us",
+                                    "latency_us_second_half", "mean_batch_images",
                                     "batches", "peak_resident_bytes", "failed", "host_loop_us", "command")}
                  for o in runs],
         "mix": runs[0]["mix"], "budget_bytes": runs[0]["budget_bytes"], "in_flight": runs[0]["in_flight"],

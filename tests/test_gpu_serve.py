@@ -299,3 +299,24 @@ def test_submitters_and_pollers_on_threads(stream):
         assert st.failedBatches == 0 and st.residentBytes == 0 and st.completed == per * submitters
     finally:
         srv.destroy()
+
+
+def test_demo_serve_leaves_warmup_out():
+    """The C++ serving driver (the benchmark's serving leg): warm-up requests are
+    served but left out of the reported figures."""
+    import json
+    import subprocess
+
+    from vk_compute_mipmaps_amd import build
+
+    cmd = [build.DEMO_PATH, "serve", "--rate", "20000", "--count", "1500", "--warmup", "500", "--seed", "9",
+           "--budget-gb", "8"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = json.loads(r.stdout)
+    assert out["requests"] == 1500 and out["warmup_requests"] == 500 and out["failed"] == 0
+    # 1500 measured arrivals at 20k/s span ~75 ms: the rate is the measured window's.
+    assert 0.03 < out["seconds"] < 1.0
+    assert 10000 < out["chains_per_s"] < 40000
+    assert out["batches"] >= 1 and out["mean_batch_images"] >= 1.0

@@ -13,10 +13,12 @@
 //       Times a loop of plain per-texture dispatches (eager, and captured into
 //       one hipGraph like a recorded command buffer) against one batched launch,
 //       checks that both write identical bytes, prints one JSON object.
-//   nvpro_pyramid_demo serve [--rate R] [--count N] [--seed S] [--hold-ms H] [--latency-csv PATH]
-//                            [--budget-gb B] [--in-flight K] [--max-batch M] [--packed]
-//                            [--queue | --rounds]
-//       Serving loop: N requests arrive as a Poisson stream of R per second;
+//   nvpro_pyramid_demo serve [--rate R] [--count N] [--warmup W] [--seed S] [--hold-ms H]
+//                            [--latency-csv PATH] [--budget-gb B] [--in-flight K] [--max-batch M]
+//                            [--packed] [--queue | --rounds]
+//       Serving loop: W + N requests arrive as a Poisson stream of R per second;
+//       the first W warm the GPU's clocks and the server up and are left out
+//       of every figure but the failure count;
 //       sizes from the mixed-size workload, 70% sRGBA8 / 30% RGBA16F; level 0
 //       is the seeded synthetic image, written on the device at launch.  The
 //       server admits against a B GB budget and batches what arrived; each
@@ -301,17 +303,19 @@ struct ServeArgs
   uint64_t seed = 9;
   bool packed = false;
   int mode    = -1;  // batch mode; -1: the server's default
+  uint32_t warmup        = 0;        // requests before the measured ones: served, not counted
   const char* latencyCsv = nullptr;  // per request: index, arrival s, width, height, format, latency us
 };
 
 static int serve(const ServeArgs& a)
 {
   using Clock        = std::chrono::steady_clock;
-  const auto sizes   = mixedSizes(a.count, a.seed);
-  std::vector<double> arrival(a.count);
-  std::vector<uint32_t> format(a.count);
+  const uint32_t total = a.warmup + a.count;  // requests 0 .. warmup-1 are the warm-up
+  const auto sizes     = mixedSizes(total, a.seed);
+  std::vector<double> arrival(total);
+  std::vector<uint32_t> format(total);
   double t = 0;
-  for(uint32_t k = 0; k < a.count; ++k)
+  for(uint32_t k = 0; k < total; ++k)
   {
     const double u = ((double)(hash4(a.seed, 3, k, k) >> 11) + 0.5) * 0x1p-53;
     t += -log(u) / a.rate;  // exponential inter-arrival times
@@ -333,46 +337,62 @@ static int serve(const ServeArgs& a)
   std::vector<NvproPyramidCompletion> done(4096);
   std::vector<double> latency, wait;
   std::vector<std::pair<double, uint64_t>> holding;  // (release time, ticket)
-  std::unordered_map<uint64_t, uint32_t> indexOf;      // ticket -> request index (latencyCsv)
+  std::unordered_map<uint64_t, uint32_t> indexOf;      // ticket -> request index
   std::vector<std::pair<uint32_t, double>> order;      // (request index, latency us)
   uint64_t bytes = 0;
-  uint32_t next = 0, finished = 0, failed = 0;
+  uint32_t next = 0, finished = 0, measured = 0, failed = 0;
   const Clock::time_point t0 = Clock::now();
-  double lastDone            = 0;
-  double lastIter = 0, gapMax = 0, pollMax = 0;  // host loop: longest gap between iterations, longest poll
+  const double start         = arrival[a.warmup];  // the measured window opens with the first measured arrival
+  double lastDone            = start;
+  double lastIter = 0, gapMax = 0, pollMax = 0;  // host loop in the window: longest gap between iterations, longest poll
   uint32_t slowPolls = 0;                        // polls over 1 ms
-  while(finished < a.count)
+  NvproPyramidServerStats atStart{};             // server counters when the window opens
+  bool opened = a.warmup == 0;
+  while(finished < total)
   {
     const double now = std::chrono::duration<double>(Clock::now() - t0).count();
-    while(next < a.count && arrival[next] <= now)
+    while(next < total && arrival[next] <= now)
     {
+      if(next == a.warmup && a.warmup != 0)
+      {
+        CHECK_PYR(nvproPyramidServerGetStats(srv, &atStart));
+        opened = true;
+      }
       NvproPyramidRequest r{format[next], sizes[next].first, sizes[next].second, 0, a.seed * 100000u + next,
                             nullptr};
       uint64_t ticket;
       CHECK_PYR(nvproPyramidServerSubmit(srv, &r, &ticket));
-      if(a.latencyCsv)
-        indexOf[ticket] = next;
+      indexOf[ticket] = next;
       ++next;
     }
     uint32_t n = 0;
     const Clock::time_point p0 = Clock::now();
     CHECK_PYR(nvproPyramidServerPoll(srv, done.data(), (uint32_t)done.size(), &n));
     const double pollUs = std::chrono::duration<double, std::micro>(Clock::now() - p0).count();
-    pollMax             = std::max(pollMax, pollUs);
-    slowPolls += pollUs > 1000.0 ? 1u : 0u;
-    const double gapUs = (now - lastIter) * 1e6;
-    gapMax             = std::max(gapMax, gapUs);
-    lastIter           = now;
+    if(opened)
+    {
+      pollMax = std::max(pollMax, pollUs);
+      slowPolls += pollUs > 1000.0 ? 1u : 0u;
+      if(lastIter >= start)
+        gapMax = std::max(gapMax, (now - lastIter) * 1e6);
+    }
+    lastIter = now;
     for(uint32_t i = 0; i < n; ++i)
     {
-      latency.push_back(done[i].latencyUs);
-      if(a.latencyCsv)
-        order.push_back({indexOf[done[i].ticket], done[i].latencyUs});
-      wait.push_back(done[i].waitUs);
-      bytes += done[i].bytes;
+      const auto it     = indexOf.find(done[i].ticket);
+      const uint32_t ix = it->second;
+      indexOf.erase(it);
       failed += done[i].status != 0 ? 1u : 0u;
       holding.push_back({now + a.holdMs * 1e-3, done[i].ticket});
       ++finished;
+      if(ix < a.warmup)
+        continue;
+      latency.push_back(done[i].latencyUs);
+      if(a.latencyCsv)
+        order.push_back({ix, done[i].latencyUs});
+      wait.push_back(done[i].waitUs);
+      bytes += done[i].bytes;
+      ++measured;
       lastDone = now;
     }
     size_t keep = 0;
@@ -390,7 +410,9 @@ static int serve(const ServeArgs& a)
   for(const auto& h : holding)
     CHECK_PYR(nvproPyramidServerRelease(srv, h.second));
   CHECK_PYR(nvproPyramidServerDestroy(srv));
-  const double span = lastDone;
+  const double span         = lastDone - start;
+  const uint64_t batches    = st.batches - atStart.batches;
+  const uint64_t batchedImg = st.batchedImages - atStart.batchedImages;
   if(a.latencyCsv)
   {
     FILE* f = fopen(a.latencyCsv, "w");
@@ -403,17 +425,18 @@ static int serve(const ServeArgs& a)
               format[o.first], o.second);
     fclose(f);
   }
-  printf("{\"requests\": %u, \"rate_per_s\": %.1f, \"mix\": \"70%% sRGBA8 / 30%% RGBA16F (%s)\", "
+  printf("{\"requests\": %u, \"warmup_requests\": %u, \"rate_per_s\": %.1f, "
+         "\"mix\": \"70%% sRGBA8 / 30%% RGBA16F (%s)\", "
          "\"budget_bytes\": %llu, \"in_flight\": %u, \"hold_ms\": %.1f, \"batch_mode\": \"%s\",\n",
-         a.count, a.rate, a.packed ? "packed_fp16" : "fp32", (unsigned long long)cfg.budgetBytes, a.inFlight,
+         a.count, a.warmup, a.rate, a.packed ? "packed_fp16" : "fp32", (unsigned long long)cfg.budgetBytes, a.inFlight,
          a.holdMs, cfg.batch.mode == NVPRO_PYRAMID_BATCH_MODE_QUEUE ? "queue" : "rounds");
   printf(" \"seconds\": %.4f, \"chains_per_s\": %.1f, \"GBps\": %.1f, \"failed\": %u,\n", span,
-         a.count / span, bytes / span / 1e9, failed);
+         measured / span, bytes / span / 1e9, failed);
   printf(" \"latency_us\": {\"p50\": %.1f, \"p90\": %.1f, \"p99\": %.1f, \"max\": %.1f}, "
          "\"admission_wait_us\": {\"p50\": %.1f, \"p99\": %.1f},\n",
          percentile(latency, 0.5), percentile(latency, 0.9), percentile(latency, 0.99), percentile(latency, 1.0),
          percentile(wait, 0.5), percentile(wait, 0.99));
-  // The second half of the completions alone: steady state, without the start.
+  // The second half of the measured completions alone.
   const std::vector<double> late(latency.begin() + latency.size() / 2, latency.end());
   printf(" \"latency_us_second_half\": {\"p50\": %.1f, \"p99\": %.1f, \"max\": %.1f},\n", percentile(late, 0.5),
          percentile(late, 0.99), percentile(late, 1.0));
@@ -421,7 +444,7 @@ static int serve(const ServeArgs& a)
          slowPolls);
   printf(" \"batches\": %llu, \"mean_batch_images\": %.2f, \"batch_image_cap\": %u, "
          "\"peak_resident_bytes\": %llu}\n",
-         (unsigned long long)st.batches, st.batches ? (double)st.batchedImages / st.batches : 0.0,
+         (unsigned long long)batches, batches ? (double)batchedImg / batches : 0.0,
          cfg.maxBatchImages, (unsigned long long)st.peakResidentBytes);
   return failed == 0 ? 0 : 1;
 }
@@ -438,6 +461,8 @@ int main(int argc, char** argv)
         a.rate = atof(argv[++i]);
       else if(o == "--count" && i + 1 < argc)
         a.count = (uint32_t)atoi(argv[++i]);
+      else if(o == "--warmup" && i + 1 < argc)
+        a.warmup = (uint32_t)atoi(argv[++i]);
       else if(o == "--seed" && i + 1 < argc)
         a.seed = (uint64_t)atoll(argv[++i]);
       else if(o == "--hold-ms" && i + 1 < argc)
@@ -462,7 +487,7 @@ int main(int argc, char** argv)
         return 4;
       }
     }
-    if(a.count == 0 || !(a.rate > 0))
+    if(a.count == 0 || !(a.rate > 0) || a.warmup > (1u << 24) || a.count > (1u << 24))
       return 4;
     return serve(a);
   }
