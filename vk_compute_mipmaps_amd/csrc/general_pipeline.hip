@@ -5,11 +5,11 @@
 
 namespace nvpro_pyramid {
 
-template <class F>
+template <class F, bool THIN>
 struct GeneralShared
 {
   typename F::Tables tab;
-  typename F::Val mid[gen2MidTexels<F>()];
+  typename F::Val mid[gen2MidTexels<F, THIN>()];
 };
 
 template <class F>
@@ -24,18 +24,19 @@ __global__ void __launch_bounds__(kGroupBS) generalOneLevelKernel(ImageDesc img,
     generalOneGroup<F>(tab, img, inLevel, g);
 }
 
-// KX, KY: the first level's kernel sizes, or 0 (read from the image).
-template <class F, int KX, int KY>
+// KX, KY: the first level's kernel sizes, or 0 (read from the image).  THIN:
+// the tile geometry (Gen2Geom).
+template <class F, int KX, int KY, bool THIN>
 __global__ void __launch_bounds__(kGroupBS) generalTwoLevelKernel(ImageDesc img, uint32_t inLevel, uint32_t tilesX,
                                                                 uint32_t numTiles)
 {
-  __shared__ GeneralShared<F> sh;
+  __shared__ GeneralShared<F, THIN> sh;
   const uint32_t lines = prefetchLevels(img);
   F::template fill<kGroupBS>(sh.tab, 1.0f);
   __syncthreads();
   levelsReady(lines);
   for(uint32_t tile = blockIdx.x; tile < numTiles; tile += gridDim.x)
-    generalTwoGroup<F, false, KX, KY>(sh.tab, sh.mid, img, inLevel, tilesX, tile);
+    generalTwoGroup<F, false, KX, KY, THIN>(sh.tab, sh.mid, img, inLevel, tilesX, tile);
 }
 
 // Workgroups of `kernel` that are resident on one compute unit at once.  A
@@ -50,26 +51,39 @@ static uint32_t residentPerCU(K kernel)
   return (uint32_t)n;
 }
 
+// A step whose thin tiles all fit on the GPU at once runs them (its time is
+// then one tile's latency, which thin tiles halve); a larger one runs the full
+// tiles, which recompute fewer border texels, on persistent workgroups.
 template <class F, int KX, int KY>
-static void launchTwoLevel(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t tilesX,
-                           uint32_t numTiles)
+static void launchTwoLevel(hipStream_t stream, const ImageDesc& img, uint32_t inLevel)
 {
-  static const uint32_t fit = residentPerCU(generalTwoLevelKernel<F, KX, KY>);
-  const uint64_t cap        = (uint64_t)deviceComputeUnits() * fit;
-  const uint32_t blocks     = (uint32_t)(numTiles < cap ? numTiles : cap);
-  hipLaunchKernelGGL((generalTwoLevelKernel<F, KX, KY>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
-                     inLevel, tilesX, numTiles);
+  static const uint32_t fit     = residentPerCU(generalTwoLevelKernel<F, KX, KY, false>);
+  static const uint32_t fitThin = residentPerCU(generalTwoLevelKernel<F, KX, KY, true>);
+  const LevelDesc& d            = img.lv[inLevel + 2u];
+  uint32_t tilesX = 0, thinX = 0;
+  const uint32_t numTiles = generalGroupCount(d.width, d.height, 2u, F::kBytes, &tilesX);
+  const uint32_t numThin  = generalGroupCount(d.width, d.height, 2u, F::kBytes, &thinX, true);
+  if((uint64_t)numThin <= (uint64_t)deviceComputeUnits() * fitThin)
+  {
+    hipLaunchKernelGGL((generalTwoLevelKernel<F, KX, KY, true>), dim3(numThin), dim3(kGroupBS), 0, stream, img,
+                       inLevel, thinX, numThin);
+    return;
+  }
+  const uint64_t cap    = (uint64_t)deviceComputeUnits() * fit;
+  const uint32_t blocks = (uint32_t)(numTiles < cap ? numTiles : cap);
+  hipLaunchKernelGGL((generalTwoLevelKernel<F, KX, KY, false>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream,
+                     img, inLevel, tilesX, numTiles);
 }
 
 template <class F>
 static hipError_t launchGeneralTyped(hipStream_t stream, const ImageDesc& img, uint32_t inLevel, uint32_t count)
 {
-  static const uint32_t oneFit = residentPerCU(generalOneLevelKernel<F>);
-  const LevelDesc& d           = img.lv[inLevel + count];
-  uint32_t tilesX              = 0;
-  const uint64_t want          = generalGroupCount(d.width, d.height, count, F::kBytes, &tilesX);
   if(count == 1u)
   {
+    static const uint32_t oneFit = residentPerCU(generalOneLevelKernel<F>);
+    const LevelDesc& d           = img.lv[inLevel + 1u];
+    uint32_t unused              = 0;
+    const uint64_t want          = generalGroupCount(d.width, d.height, 1u, F::kBytes, &unused);
     const uint64_t cap    = (uint64_t)deviceComputeUnits() * oneFit;
     const uint32_t blocks = (uint32_t)(want < cap ? want : cap);
     hipLaunchKernelGGL((generalOneLevelKernel<F>), dim3(blocks ? blocks : 1), dim3(kGroupBS), 0, stream, img,
@@ -79,15 +93,15 @@ static hipError_t launchGeneralTyped(hipStream_t stream, const ImageDesc& img, u
   // Two levels: the kernel specialised for the first level's kernel sizes.
   const int kx = kernelSize(img.lv[inLevel].width), ky = kernelSize(img.lv[inLevel].height);
   if(kx == 3 && ky == 3)
-    launchTwoLevel<F, 3, 3>(stream, img, inLevel, tilesX, (uint32_t)want);
+    launchTwoLevel<F, 3, 3>(stream, img, inLevel);
   else if(kx == 2 && ky == 2)
-    launchTwoLevel<F, 2, 2>(stream, img, inLevel, tilesX, (uint32_t)want);
+    launchTwoLevel<F, 2, 2>(stream, img, inLevel);
   else if(kx == 3 && ky == 2)
-    launchTwoLevel<F, 3, 2>(stream, img, inLevel, tilesX, (uint32_t)want);
+    launchTwoLevel<F, 3, 2>(stream, img, inLevel);
   else if(kx == 2 && ky == 3)
-    launchTwoLevel<F, 2, 3>(stream, img, inLevel, tilesX, (uint32_t)want);
+    launchTwoLevel<F, 2, 3>(stream, img, inLevel);
   else
-    launchTwoLevel<F, 0, 0>(stream, img, inLevel, tilesX, (uint32_t)want);
+    launchTwoLevel<F, 0, 0>(stream, img, inLevel);
   return hipGetLastError();
 }
 

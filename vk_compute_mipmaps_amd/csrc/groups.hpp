@@ -307,13 +307,16 @@ __device__ __forceinline__ void fastGroup(const TT& tab, typename F::Val* level2
 // lane c of every wave owns first-level column c of the tile (MW = 63, so a
 // wave spans the tile's width and lane 63 is a helper), and wave w owns the
 // first-level rows [w R, w R + R).  Narrow texels take 7 second-level rows;
-// 16-byte texels 5, to bound the registers of a lane's input window.
-template <uint32_t BYTES>
+// 16-byte texels 5, to bound the registers of a lane's input window.  THIN
+// tiles (3 second-level rows, 2 first-level rows per wave) are for steps too
+// small to fill the GPU: each lane then runs half the serial work, and a
+// step's latency is one tile's.
+template <uint32_t BYTES, bool THIN = false>
 struct Gen2Geom
 {
   static constexpr uint32_t TX = 31;
-  static constexpr uint32_t TY = BYTES <= 8u ? 7u : 5u;
-  static constexpr uint32_t R  = BYTES <= 8u ? 4u : 3u;
+  static constexpr uint32_t TY = THIN ? 3u : (BYTES <= 8u ? 7u : 5u);
+  static constexpr uint32_t R  = THIN ? 2u : (BYTES <= 8u ? 4u : 3u);
   static constexpr uint32_t MW = 2u * TX + 1u;  // LDS row stride of the first level
   static constexpr uint32_t MH = 2u * TY + 1u;
   static_assert(MW == 63u, "one lane per first-level column, one wave across the tile");
@@ -322,18 +325,19 @@ struct Gen2Geom
 };
 
 // Groups of a one-level general step (a thread per output) and of a two-level
-// one (a Gen2Geom tile of the second level each; *tilesX receives the tile
-// columns).
+// one (a Gen2Geom tile of the second level each, thin or not; *tilesX receives
+// the tile columns).
 __host__ __device__ inline uint32_t generalGroupCount(uint32_t dstWidth, uint32_t dstHeight, uint32_t count,
-                                                      uint32_t bytesPerTexel, uint32_t* tilesX)
+                                                      uint32_t bytesPerTexel, uint32_t* tilesX, bool thin = false)
 {
   if(count == 1u)
   {
     *tilesX = 0;
     return (uint32_t)(((uint64_t)dstWidth * dstHeight + kGroupBS - 1) / kGroupBS);
   }
-  const uint32_t tx = bytesPerTexel <= 8u ? Gen2Geom<8>::TX : Gen2Geom<16>::TX;
-  const uint32_t ty = bytesPerTexel <= 8u ? Gen2Geom<8>::TY : Gen2Geom<16>::TY;
+  const uint32_t tx = Gen2Geom<8>::TX;
+  static_assert(Gen2Geom<8>::TX == Gen2Geom<16>::TX && Gen2Geom<8, true>::TX == Gen2Geom<16, true>::TX, "one width");
+  const uint32_t ty = thin ? Gen2Geom<8, true>::TY : bytesPerTexel <= 8u ? Gen2Geom<8>::TY : Gen2Geom<16>::TY;
   *tilesX           = (dstWidth + tx - 1u) / tx;
   return *tilesX * ((dstHeight + ty - 1u) / ty);
 }
@@ -359,10 +363,10 @@ __device__ __forceinline__ void generalOneGroup(const TT& tab, const ImageDesc& 
 }
 
 // LDS values of a two-level general group.
-template <class F>
+template <class F, bool THIN = false>
 constexpr uint32_t gen2MidTexels()
 {
-  return Gen2Geom<F::kBytes>::MW * Gen2Geom<F::kBytes>::MH;
+  return Gen2Geom<F::kBytes, THIN>::MW * Gen2Geom<F::kBytes, THIN>::MH;
 }
 
 // Two levels: the Gen2Geom tile `tile` of level inLevel+2 and the first-level
@@ -384,10 +388,10 @@ constexpr uint32_t gen2MidTexels()
 // Second level (fillLastTile_, glsl:791-820): from LDS, a thread per texel.
 // Neighbouring tiles recompute their shared first-level border texels and
 // store identical values.
-template <class F>
+template <class F, bool THIN = false>
 struct Gen2Window
 {
-  typename F::Raw raw[2u * Gen2Geom<F::kBytes>::R + 1u][2];
+  typename F::Raw raw[2u * Gen2Geom<F::kBytes, THIN>::R + 1u][2];
 };
 
 // Lane i receives lane i+1's value (DPP wave_shl:1; lane 63 keeps its own).
@@ -415,10 +419,10 @@ struct Gen2Lane
 // KX, KY: the first level's kernel sizes (kernelSize of the input width and
 // height) when known at compile time, 0 to read them from the image.  Each
 // specialisation computes exactly what the general one does.
-template <class F, int KX = 0>
+template <class F, int KX = 0, bool THIN = false>
 __device__ __forceinline__ Gen2Lane gen2Lane(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile)
 {
-  using G              = Gen2Geom<F::kBytes>;
+  using G              = Gen2Geom<F::kBytes, THIN>;
   const LevelDesc& src = img.lv[inLevel];
   const LevelDesc& l1  = img.lv[inLevel + 1u];
   const int kx1 = KX ? KX : kernelSize(src.width);
@@ -443,18 +447,18 @@ __device__ __forceinline__ Gen2Lane gen2Lane(const ImageDesc& img, uint32_t inLe
   return r;
 }
 
-template <class F, int KX = 0, int KY = 0>
+template <class F, int KX = 0, int KY = 0, bool THIN = false>
 __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile,
-                                         Gen2Window<F>& w)
+                                         Gen2Window<F, THIN>& w)
 {
-  constexpr uint32_t NIN = 2u * Gen2Geom<F::kBytes>::R + 1u;
+  constexpr uint32_t NIN = 2u * Gen2Geom<F::kBytes, THIN>::R + 1u;
 #pragma unroll
   for(uint32_t r = 0; r < NIN; ++r)
   {
     w.raw[r][0] = typename F::Raw{};
     w.raw[r][1] = typename F::Raw{};
   }
-  const Gen2Lane L = gen2Lane<F, KX>(img, inLevel, tilesX, tile);
+  const Gen2Lane L = gen2Lane<F, KX, THIN>(img, inLevel, tilesX, tile);
   if(L.n == 0u)
     return;
   const LevelDesc& src = img.lv[inLevel];
@@ -480,11 +484,11 @@ __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel,
 // 2 ox + 2 from lane + 1 (its column 2 (ox + 1)); the helper (ox = TX) only
 // lends its column.  Per output these are generalSample's operations on the
 // same values, from six LDS reads instead of nine.
-template <class F, bool CO, int KY2, class TT>
+template <class F, bool CO, int KY2, bool THIN, class TT>
 __device__ __forceinline__ void gen2SecondLevel3(const TT& tab, const typename F::Val* mid, const ImageDesc& img,
                                                  uint32_t inLevel, uint32_t tilesX, uint32_t tile, int ky2)
 {
-  using G = Gen2Geom<F::kBytes>;
+  using G = Gen2Geom<F::kBytes, THIN>;
   using V = typename F::Val;
   static_assert(G::TX + 1u == 32u, "a row of outputs and its helper fill half a wave");
   const int ky = KY2 ? KY2 : ky2;
@@ -516,18 +520,19 @@ __device__ __forceinline__ void gen2SecondLevel3(const TT& tab, const typename F
 }
 
 // Contains barriers.  CO: the second level is stored with storeTexelCoherent.
-template <class F, bool CO = false, int KX = 0, int KY = 0, class TT>
+template <class F, bool CO = false, int KX = 0, int KY = 0, bool THIN = false, class TT>
 __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid, const ImageDesc& img,
-                                            uint32_t inLevel, uint32_t tilesX, uint32_t tile, const Gen2Window<F>& w)
+                                            uint32_t inLevel, uint32_t tilesX, uint32_t tile,
+                                            const Gen2Window<F, THIN>& w)
 {
-  using G              = Gen2Geom<F::kBytes>;
+  using G              = Gen2Geom<F::kBytes, THIN>;
   using V              = typename F::Val;
   const LevelDesc& src = img.lv[inLevel];
   const LevelDesc& l1  = img.lv[inLevel + 1u];
   const LevelDesc& l2  = img.lv[inLevel + 2u];
   const int kx1 = KX ? KX : kernelSize(src.width), ky1 = KY ? KY : kernelSize(src.height);
   const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
-  const Gen2Lane L = gen2Lane<F, KX>(img, inLevel, tilesX, tile);
+  const Gen2Lane L = gen2Lane<F, KX, THIN>(img, inLevel, tilesX, tile);
   if(L.n != 0u)  // the same in the whole wave: the wave shift below sees every lane
   {
     const Axis3 ax = axis3(l1.width), ay = axis3(l1.height);
@@ -567,11 +572,11 @@ __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid,
   if(kx2 == 3)  // the same for every tile of a step
   {
     if(ky2 == 3)
-      gen2SecondLevel3<F, CO, 3>(tab, mid, img, inLevel, tilesX, tile, ky2);
+      gen2SecondLevel3<F, CO, 3, THIN>(tab, mid, img, inLevel, tilesX, tile, ky2);
     else if(ky2 == 2)
-      gen2SecondLevel3<F, CO, 2>(tab, mid, img, inLevel, tilesX, tile, ky2);
+      gen2SecondLevel3<F, CO, 2, THIN>(tab, mid, img, inLevel, tilesX, tile, ky2);
     else
-      gen2SecondLevel3<F, CO, 0>(tab, mid, img, inLevel, tilesX, tile, ky2);
+      gen2SecondLevel3<F, CO, 0, THIN>(tab, mid, img, inLevel, tilesX, tile, ky2);
   }
   else if(threadIdx.x < G::TX * G::TY)
   {
@@ -589,13 +594,13 @@ __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid,
 }
 
 // One tile.  Contains barriers.
-template <class F, bool CO = false, int KX = 0, int KY = 0, class TT>
+template <class F, bool CO = false, int KX = 0, int KY = 0, bool THIN = false, class TT>
 __device__ __forceinline__ void generalTwoGroup(const TT& tab, typename F::Val* mid, const ImageDesc& img,
                                                 uint32_t inLevel, uint32_t tilesX, uint32_t tile)
 {
-  Gen2Window<F> w;
-  gen2Load<F, KX, KY>(img, inLevel, tilesX, tile, w);
-  gen2Compute<F, CO, KX, KY>(tab, mid, img, inLevel, tilesX, tile, w);
+  Gen2Window<F, THIN> w;
+  gen2Load<F, KX, KY, THIN>(img, inLevel, tilesX, tile, w);
+  gen2Compute<F, CO, KX, KY, THIN>(tab, mid, img, inLevel, tilesX, tile, w);
 }
 
 // One tile, through the specialisation for the first level's kernel sizes
