@@ -394,7 +394,10 @@ struct Gen2Window
   typename F::Raw raw[2u * Gen2Geom<F::kBytes, THIN>::R + 1u][2];
 };
 
-// Lane i receives lane i+1's value (DPP wave_shl:1; lane 63 keeps its own).
+// Lane i receives lane i+1's value (DPP wave_shl:1); lane 63 receives 0, and
+// no caller uses it (it is a helper lane).  With no old value to keep, the
+// compiler folds the shift into the instruction that consumes it (a DPP
+// operand of the FMA).
 template <class V>
 __device__ __forceinline__ V waveShiftDown1(V v)
 {
@@ -403,7 +406,7 @@ __device__ __forceinline__ V waveShiftDown1(V v)
   __builtin_memcpy(w, &v, sizeof v);
 #pragma unroll
   for(int k = 0; k < (int)(sizeof(V) / 4); ++k)
-    w[k] = (uint32_t)__builtin_amdgcn_update_dpp((int)w[k], (int)w[k], 0x130, 0xf, 0xf, false);
+    w[k] = (uint32_t)__builtin_amdgcn_mov_dpp((int)w[k], 0x130, 0xf, 0xf, true);
   V r;
   __builtin_memcpy(&r, w, sizeof r);
   return r;
@@ -452,27 +455,28 @@ __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel,
                                          Gen2Window<F, THIN>& w)
 {
   constexpr uint32_t NIN = 2u * Gen2Geom<F::kBytes, THIN>::R + 1u;
-#pragma unroll
-  for(uint32_t r = 0; r < NIN; ++r)
-  {
-    w.raw[r][0] = typename F::Raw{};
-    w.raw[r][1] = typename F::Raw{};
-  }
-  const Gen2Lane L = gen2Lane<F, KX, THIN>(img, inLevel, tilesX, tile);
-  if(L.n == 0u)
-    return;
-  const LevelDesc& src = img.lv[inLevel];
-  const uint32_t rows  = 2u * (L.n - 1u) + (uint32_t)(KY ? KY : kernelSize(src.height));  // input rows the lane reads
-  const uint8_t* p = img.base + src.offset + (uint64_t)(2u * L.gy0) * src.pitch + (uint64_t)(2u * L.gx) * F::kBytes;
+  const Gen2Lane L       = gen2Lane<F, KX, THIN>(img, inLevel, tilesX, tile);
+  const LevelDesc& src   = img.lv[inLevel];
+  // Input rows the wave reads (the same for all its lanes; in the level).
+  const uint32_t rows = L.n == 0u ? 0u : 2u * (L.n - 1u) + (uint32_t)(KY ? KY : kernelSize(src.height));
+  // Every lane loads both of its columns, clamped into the level, rather than
+  // branching per load: a lane that neither computes a texel nor lends its
+  // first column (L.c0, L.c1 false) reads texels whose values go unused.
+  const uint32_t last = src.width - 1u;
+  const uint32_t x0 = 2u * L.gx < last ? 2u * L.gx : last, x1 = 2u * L.gx + 1u < last ? 2u * L.gx + 1u : last;
+  const uint8_t* p  = img.base + src.offset + (uint64_t)(2u * L.gy0) * src.pitch;
 #pragma unroll
   for(uint32_t r = 0; r < NIN; ++r)
   {
     if(r < rows)
     {
-      if(L.c0)
-        w.raw[r][0] = loadTexel<F>(p + (uint64_t)r * src.pitch);
-      if(L.c1)
-        w.raw[r][1] = loadTexel<F>(p + (uint64_t)r * src.pitch + F::kBytes);
+      w.raw[r][0] = loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)x0 * F::kBytes);
+      w.raw[r][1] = loadTexel<F>(p + (uint64_t)r * src.pitch + (uint64_t)x1 * F::kBytes);
+    }
+    else
+    {
+      w.raw[r][0] = typename F::Raw{};
+      w.raw[r][1] = typename F::Raw{};
     }
   }
 }
