@@ -346,6 +346,7 @@ static int serve(const ServeArgs& a)
   double lastDone            = start;
   double lastIter = 0, gapMax = 0, pollMax = 0;  // host loop in the window: longest gap between iterations, longest poll
   uint32_t slowPolls = 0;                        // polls over 1 ms
+  double pollSum = 0, submitSum = 0;             // host time in the window inside Poll and Submit, seconds
   NvproPyramidServerStats atStart{};             // server counters when the window opens
   bool opened = a.warmup == 0;
   while(finished < total)
@@ -361,7 +362,10 @@ static int serve(const ServeArgs& a)
       NvproPyramidRequest r{format[next], sizes[next].first, sizes[next].second, 0, a.seed * 100000u + next,
                             nullptr};
       uint64_t ticket;
+      const Clock::time_point s0 = Clock::now();
       CHECK_PYR(nvproPyramidServerSubmit(srv, &r, &ticket));
+      if(opened)
+        submitSum += std::chrono::duration<double>(Clock::now() - s0).count();
       indexOf[ticket] = next;
       ++next;
     }
@@ -373,6 +377,7 @@ static int serve(const ServeArgs& a)
     {
       pollMax = std::max(pollMax, pollUs);
       slowPolls += pollUs > 1000.0 ? 1u : 0u;
+      pollSum += pollUs * 1e-6;
       if(lastIter >= start)
         gapMax = std::max(gapMax, (now - lastIter) * 1e6);
     }
@@ -440,8 +445,9 @@ static int serve(const ServeArgs& a)
   const std::vector<double> late(latency.begin() + latency.size() / 2, latency.end());
   printf(" \"latency_us_second_half\": {\"p50\": %.1f, \"p99\": %.1f, \"max\": %.1f},\n", percentile(late, 0.5),
          percentile(late, 0.99), percentile(late, 1.0));
-  printf(" \"host_loop_us\": {\"max_gap\": %.1f, \"max_poll\": %.1f, \"polls_over_1ms\": %u},\n", gapMax, pollMax,
-         slowPolls);
+  printf(" \"host_loop_us\": {\"max_gap\": %.1f, \"max_poll\": %.1f, \"polls_over_1ms\": %u, "
+         "\"poll_busy_frac\": %.3f, \"submit_busy_frac\": %.3f},\n",
+         gapMax, pollMax, slowPolls, pollSum / span, submitSum / span);
   printf(" \"batches\": %llu, \"mean_batch_images\": %.2f, \"batch_image_cap\": %u, "
          "\"peak_resident_bytes\": %llu}\n",
          (unsigned long long)batches, batches ? (double)batchedImg / batches : 0.0,
