@@ -124,8 +124,8 @@ __device__ __forceinline__ void storeTexel(uint8_t* p, typename F::Raw v)
 // level: otherwise each level's first use that falls in a new 64-byte line
 // misses and stalls the wave there (about 0.3 us on MI355X), which a kernel
 // that walks many small levels pays once per line.  Call at kernel start and
-// pass the result to levelsReady() after the table fill's barrier, so that the
-// loads' latency overlaps the fill.
+// pass the result to levelsReady() after the table fill's barrier: the loads
+// then all miss at once, in one round, before the first level is addressed.
 __device__ __forceinline__ uint32_t prefetchLevels(const ImageDesc& img)
 {
   uint32_t acc = img.levelCount;
