@@ -150,7 +150,9 @@ uint32_t nvproPyramidDefaultGeneralDispatcher(NvproPyramidCmd* cmd, const NvproP
  * or the end of the process, and while 64 streams of a device hold one, others
  * do not fuse); it is off while the stream is being
  * captured into a graph, and when the environment variable NVPRO_PYRAMID_FUSE
- * is "0". */
+ * is "0".  NVPRO_PYRAMID_THIN=1 / =0 forces the general steps' thin (31x3) or
+ * full (31x7) tiles; by default a step runs thin tiles when all of them fit on
+ * the GPU at once.  Neither changes the bytes written. */
 int nvproPyramidDispatch(hipStream_t stream, const NvproPyramidPipelines* pipelines,
                          const NvproPyramidImage* image, uint32_t mipLevels);
 

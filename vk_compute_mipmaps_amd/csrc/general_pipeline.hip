@@ -1,6 +1,8 @@
 // General pipeline kernels (nvpro_pyramid.glsl:536-884) for gfx950: one launch
 // per general step of 1 or 2 levels.  The group bodies (generalOneGroup,
 // generalTwoGroup) are in groups.hpp.  Also the per-step executor.
+#include <stdlib.h>
+
 #include "groups.hpp"
 
 namespace nvpro_pyramid {
@@ -53,7 +55,9 @@ static uint32_t residentPerCU(K kernel)
 
 // A step whose thin tiles all fit on the GPU at once runs them (its time is
 // then one tile's latency, which thin tiles halve); a larger one runs the full
-// tiles, which recompute fewer border texels, on persistent workgroups.
+// tiles, which recompute fewer border texels, on persistent workgroups.  The
+// environment variable NVPRO_PYRAMID_THIN=1 / =0 forces thin / full tiles for
+// every step (the bytes are the same either way).
 template <class F, int KX, int KY>
 static void launchTwoLevel(hipStream_t stream, const ImageDesc& img, uint32_t inLevel)
 {
@@ -63,7 +67,11 @@ static void launchTwoLevel(hipStream_t stream, const ImageDesc& img, uint32_t in
   uint32_t tilesX = 0, thinX = 0;
   const uint32_t numTiles = generalGroupCount(d.width, d.height, 2u, F::kBytes, &tilesX);
   const uint32_t numThin  = generalGroupCount(d.width, d.height, 2u, F::kBytes, &thinX, true);
-  if((uint64_t)numThin <= (uint64_t)deviceComputeUnits() * fitThin)
+  const char* force       = getenv("NVPRO_PYRAMID_THIN");
+  const bool thin         = force && (force[0] == '0' || force[0] == '1')
+                                ? force[0] == '1'
+                                : (uint64_t)numThin <= (uint64_t)deviceComputeUnits() * fitThin;
+  if(thin)
   {
     hipLaunchKernelGGL((generalTwoLevelKernel<F, KX, KY, true>), dim3(numThin), dim3(kGroupBS), 0, stream, img,
                        inLevel, thinX, numThin);
