@@ -416,19 +416,14 @@ struct Gen2Lane
 {
   uint32_t gx, gy0, n;  // first-level column, first row of the wave's rows, rows (the same in the whole wave)
   bool out;             // the lane computes first-level texels (else it only lends column 2 gx)
-  bool c0, c1;          // input columns 2 gx and 2 gx + 1 are loaded
 };
 
-// KX, KY: the first level's kernel sizes (kernelSize of the input width and
-// height) when known at compile time, 0 to read them from the image.  Each
-// specialisation computes exactly what the general one does.
-template <class F, int KX = 0, bool THIN = false>
+// The lane's place in tile `tile` (the same for every kernel size).
+template <class F, bool THIN = false>
 __device__ __forceinline__ Gen2Lane gen2Lane(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile)
 {
-  using G              = Gen2Geom<F::kBytes, THIN>;
-  const LevelDesc& src = img.lv[inLevel];
-  const LevelDesc& l1  = img.lv[inLevel + 1u];
-  const int kx1 = KX ? KX : kernelSize(src.width);
+  using G             = Gen2Geom<F::kBytes, THIN>;
+  const LevelDesc& l1 = img.lv[inLevel + 1u];
   const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
   const uint32_t lane = threadIdx.x & 63u, ly0 = (threadIdx.x >> 6) * G::R;
   const uint32_t ty = tile / tilesX, tx = tile - ty * tilesX;
@@ -445,23 +440,26 @@ __device__ __forceinline__ Gen2Lane gen2Lane(const ImageDesc& img, uint32_t inLe
     r.n        = l1.height - r.gy0 < n ? l1.height - r.gy0 : n;
   }
   r.out = lane < nx && r.gx < l1.width;
-  r.c0  = lane <= nx && 2u * r.gx < src.width;  // own column, or lent to lane - 1
-  r.c1  = r.out && kx1 >= 2;
   return r;
 }
 
+// KX, KY: the first level's kernel sizes (kernelSize of the input width and
+// height) when known at compile time, 0 to read them from the image.  Each
+// specialisation computes exactly what the general one does.
 template <class F, int KX = 0, int KY = 0, bool THIN = false>
 __device__ __forceinline__ void gen2Load(const ImageDesc& img, uint32_t inLevel, uint32_t tilesX, uint32_t tile,
                                          Gen2Window<F, THIN>& w)
 {
   constexpr uint32_t NIN = 2u * Gen2Geom<F::kBytes, THIN>::R + 1u;
-  const Gen2Lane L       = gen2Lane<F, KX, THIN>(img, inLevel, tilesX, tile);
+  const Gen2Lane L       = gen2Lane<F, THIN>(img, inLevel, tilesX, tile);
   const LevelDesc& src   = img.lv[inLevel];
   // Input rows the wave reads (the same for all its lanes; in the level).
   const uint32_t rows = L.n == 0u ? 0u : 2u * (L.n - 1u) + (uint32_t)(KY ? KY : kernelSize(src.height));
   // Every lane loads both of its columns, clamped into the level, rather than
-  // branching per load: a lane that neither computes a texel nor lends its
-  // first column (L.c0, L.c1 false) reads texels whose values go unused.
+  // branching per load.  Only three kinds of values are used: a computing
+  // lane's two columns (the second only for a kernel 2 or 3 wide), and column
+  // 2 gx of a lane whose left neighbour computes with a 3-wide kernel, which is
+  // then inside the level.  Whatever else a lane reads goes unused.
   const uint32_t last = src.width - 1u;
   const uint32_t x0 = 2u * L.gx < last ? 2u * L.gx : last, x1 = 2u * L.gx + 1u < last ? 2u * L.gx + 1u : last;
   const uint8_t* p  = img.base + src.offset + (uint64_t)(2u * L.gy0) * src.pitch;
@@ -536,7 +534,7 @@ __device__ __forceinline__ void gen2Compute(const TT& tab, typename F::Val* mid,
   const LevelDesc& l2  = img.lv[inLevel + 2u];
   const int kx1 = KX ? KX : kernelSize(src.width), ky1 = KY ? KY : kernelSize(src.height);
   const int kx2 = kernelSize(l1.width), ky2 = kernelSize(l1.height);
-  const Gen2Lane L = gen2Lane<F, KX, THIN>(img, inLevel, tilesX, tile);
+  const Gen2Lane L = gen2Lane<F, THIN>(img, inLevel, tilesX, tile);
   if(L.n != 0u)  // the same in the whole wave: the wave shift below sees every lane
   {
     const Axis3 ax = axis3(l1.width), ay = axis3(l1.height);
