@@ -27,18 +27,24 @@ __global__ void __launch_bounds__(kGroupBS) generalOneLevelKernel(ImageDesc img,
 }
 
 // KX, KY: the first level's kernel sizes, or 0 (read from the image).  THIN:
-// the tile geometry (Gen2Geom).
+// the tile geometry (Gen2Geom) and one tile per workgroup (the grid is
+// numTiles); otherwise persistent workgroups loop over the tiles.
 template <class F, int KX, int KY, bool THIN>
 __global__ void __launch_bounds__(kGroupBS) generalTwoLevelKernel(ImageDesc img, uint32_t inLevel, uint32_t tilesX,
                                                                 uint32_t numTiles)
 {
   __shared__ GeneralShared<F, THIN> sh;
+  if(THIN && blockIdx.x >= numTiles)  // the same for the whole workgroup
+    return;
   const uint32_t lines = prefetchLevels(img);
   F::template fill<kGroupBS>(sh.tab, 1.0f);
   __syncthreads();
   levelsReady(lines);
-  for(uint32_t tile = blockIdx.x; tile < numTiles; tile += gridDim.x)
-    generalTwoGroup<F, false, KX, KY, THIN>(sh.tab, sh.mid, img, inLevel, tilesX, tile);
+  if constexpr(THIN)  // one tile per workgroup, straight-line: ~20 fewer VGPRs than the loop
+    generalTwoGroup<F, false, KX, KY, true>(sh.tab, sh.mid, img, inLevel, tilesX, blockIdx.x);
+  else
+    for(uint32_t tile = blockIdx.x; tile < numTiles; tile += gridDim.x)
+      generalTwoGroup<F, false, KX, KY, false>(sh.tab, sh.mid, img, inLevel, tilesX, tile);
 }
 
 // Workgroups of `kernel` that are resident on one compute unit at once.  A
